@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the matrix-free Stokes hot path on one MI355X.
+
+Workload (one "step"): one full epsilon-div-div operator apply -- zero-fill,
+matrix-free wedge kernel, additive interface sum -- on the velocity mesh of
+the mixed-precision Stokes configuration: lateral level 7 (129 x 129 nodes
+per diamond) x 129 radial nodes, 10 diamonds, r in [0.55, 1], 63 406 854
+velocity DoF, FP64, Dirichlet on both shells.  Inputs are resident in HBM
+before the timed region.  GDoF = 3 x owned velocity nodes (the reference's
+convention, apps/benchmarks/performance/benchmark_operators.cpp:207-212).
+
+Prints ONE JSON line (rank 0).  `roofline` is for the dominant kernel (the
+wedge kernel alone, timed with HIP events on its own stream); `cpu_baseline`
+is the CPU oracle (OpenMP C) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--lat", type=int, default=7)
+    p.add_argument("--rad", type=int, default=7)
+    p.add_argument("--dtype", default="f64", choices=["f64", "f32"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(seconds=10.0):
+    """Time the CPU oracle's FP64 apply on a bounded sample (lateral 5 x 33 radial)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+
+    import oracle_lib as O
+    from fields import consistent_random, radial_eta
+
+    O.build()
+    om = O.Mesh(5, 33, 0.55, 1.0)
+    eta = radial_eta(om.radii, (10, om.n, om.n, om.nr)).reshape(-1)
+    x = consistent_random(om, 3, 0).reshape(-1)
+    op = O.Eps(om, eta, O.DIRICHLET, O.DIRICHLET)
+    op.apply(x)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        op.apply(x)
+        reps += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    gdofs = 3 * int(om.owned.sum()) / dt / 1e9
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": round(gdofs, 5),
+        "unit": "GDoF/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle FP64 eps apply (assembled local-matrix form), lateral 5 x 33 radial "
+                  f"({3 * int(om.owned.sum())} DoF), Dirichlet, {reps} applies in {dt * reps:.1f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import terraneo_amd as tn
+
+    dev = torch.device("cuda:0")
+    tdt = torch.float64 if args.dtype == "f64" else torch.float32
+    esize = 8 if args.dtype == "f64" else 4
+
+    mesh = tn.Mesh(args.lat, args.rad, 0.55, 1.0)
+    radii = torch.from_numpy(mesh.radii()).to(dev)
+    shape = mesh.shape
+    eta = torch.exp(np.log(100.0) * (radii - 0.55) / 0.45).to(tdt).view(1, 1, 1, -1).expand(shape).contiguous()
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    x = (torch.rand((3,) + shape, generator=g, device=dev, dtype=tdt) * 2 - 1).contiguous()
+    mesh.interface_sum(x, 3)  # make duplicated nodes consistent (value irrelevant for timing)
+    y = torch.empty_like(x)
+    op = tn.Eps(mesh, eta, tn.TN_DIRICHLET, tn.TN_DIRICHLET)
+
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        op.apply(x, y)
+    torch.cuda.synchronize()
+
+    # ---- timed: full apply (fill + kernel + interface sum)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        op.apply(x, y)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+
+    # ---- dominant kernel alone: Add mode + skipped communication = one kernel launch
+    op.set_modes(tn.TN_ADD, tn.TN_COMM_SKIP)
+    for _ in range(3):
+        op.apply(x, y)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    kreps = max(10, args.steps)
+    e0.record(stream)
+    for _ in range(kreps):
+        op.apply(x, y)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    kernel_ms = e0.elapsed_time(e1) / kreps
+    op.set_modes(tn.TN_REPLACE, tn.TN_COMM_ADDITIVE)
+
+    dofs = 3 * mesh.owned_nodes
+    value = dofs / (ms_per_step * 1e-3) / 1e9
+    alg_bytes = mesh.N * (3 + 1 + 3) * esize  # read x, read eta, write y
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"eps_apply_L{args.lat}x{(1 << args.rad) + 1}_{args.dtype}_traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline()
+        except Exception as e:  # the baseline never blocks the GPU number
+            cpu = {"error": str(e)[:200]}
+
+    out = {
+        "metric": "Stokes solve s to rel. res 1e-8 (mixed vs FP64 GMG); operator apply GDoF/s",
+        "value": round(value, 3),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {
+            "workload": f"epsilon-div-div operator apply (fill + wedge kernel + interface sum), "
+                        f"lateral level {args.lat} x {(1 << args.rad) + 1} radial nodes, 10 diamonds, "
+                        f"Dirichlet/Dirichlet, eta = exp(ln100 (r-0.55)/0.45)",
+            "velocity_dof": dofs,
+            "stored_nodes": mesh.N,
+        },
+        "kernel_ms": round(kernel_ms, 4),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,78 @@
+"""ctypes declarations for include/terraneo_amd.h.
+
+Loading fails loudly when the shared library has not been built: there is no
+CPU fallback for any compute entry point.
+"""
+import ctypes as C
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libterraneo_amd.so")
+
+TN_F64, TN_F32 = 0, 1
+TN_NEUMANN, TN_DIRICHLET, TN_FREESLIP = 0, 1, 2
+TN_REPLACE, TN_ADD = 0, 1
+TN_COMM_SKIP, TN_COMM_ADDITIVE = 0, 1
+
+
+class TNError(RuntimeError):
+    pass
+
+
+_lib = None
+
+vp = C.c_void_p
+i32, i64, f64 = C.c_int, C.c_int64, C.c_double
+pvp = C.POINTER(C.c_void_p)
+
+# name: (restype, [argtypes])
+SIGNATURES = {
+    "tn_last_error": (C.c_char_p, []),
+    "tn_version": (i32, []),
+    "tn_malloc": (i32, [pvp, C.c_size_t]),
+    "tn_free": (i32, [vp]),
+    "tn_memcpy_h2d": (i32, [vp, vp, C.c_size_t]),
+    "tn_memcpy_d2h": (i32, [vp, vp, C.c_size_t]),
+    "tn_sync": (i32, []),
+    "tn_mesh_create": (i32, [i32, i32, f64, f64, pvp]),
+    "tn_mesh_create_radii": (i32, [i32, i32, C.POINTER(f64), pvp]),
+    "tn_mesh_destroy": (i32, [vp]),
+    "tn_mesh_dims": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]),
+    "tn_mesh_owned_nodes": (i64, [vp]),
+    "tn_mesh_lateral_coords": (i32, [vp, vp]),
+    "tn_mesh_radii": (i32, [vp, vp]),
+    "tn_mesh_owned_mask": (i32, [vp, vp]),
+    "tn_mesh_interface_info": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
+    "tn_interface_sum": (i32, [vp, i32, i32, vp, vp]),
+    "tn_lincomb": (i32, [vp, i32, i32, vp, i32, C.POINTER(f64), pvp, f64, vp]),
+    "tn_dot": (i32, [vp, i32, i32, vp, vp, C.POINTER(f64), vp]),
+    "tn_eps_create": (i32, [vp, i32, vp, i32, i32, pvp]),
+    "tn_eps_destroy": (i32, [vp]),
+    "tn_eps_set_modes": (i32, [vp, i32, i32]),
+    "tn_eps_set_diagonal": (i32, [vp, i32]),
+    "tn_eps_set_boundary_conditions": (i32, [vp, i32, i32]),
+    "tn_eps_set_eta": (i32, [vp, vp]),
+    "tn_eps_apply": (i32, [vp, vp, vp, vp]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TNError(
+                f"{LIB_PATH} is missing: build it with `python -m terraneo_amd.build` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().tn_last_error()
+        raise TNError(msg.decode() if msg else f"terraneo_amd error {rc}")
+    return rc

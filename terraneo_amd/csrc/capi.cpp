@@ -1,0 +1,361 @@
+// C ABI: mesh, level-1 operations and the epsilon-div-div operator.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "tn_internal.hpp"
+
+namespace tn {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace tn
+
+#define TN_API_BEGIN try {
+#define TN_API_END                          \
+    }                                       \
+    catch (const std::exception& e)         \
+    {                                       \
+        tn::set_last_error(e.what());       \
+        return -1;                          \
+    }                                       \
+    return 0;
+
+using tn::MeshLevel;
+
+// ------------------------------------------------------------------ epsilon operator object
+struct tn_eps {
+    const tn_mesh* mesh = nullptr;
+    int dtype = TN_F64;
+    const void* eta = nullptr;
+    int bc_cmb = TN_NEUMANN, bc_surf = TN_NEUMANN;
+    int apply_mode = TN_REPLACE, comm_mode = TN_COMM_ADDITIVE;
+    bool diagonal = false;
+    double penalty_eps = 1.0;
+    // fs/fs rotation penalty: y += eps * (G^{-1} beta) x p, beta = sum_owned p x u
+    tn::DevBuf<double> pen;  // [0..3) beta, [4..13) S = eps G^{-1}
+    bool penalty_ready = false;
+};
+
+namespace {
+
+void build_penalty(tn_eps* op)
+{
+    const MeshLevel& m = op->mesh->lvl;
+    // Gram matrix of the rotation fields e_j x p over owned nodes:
+    // G = sum (|p|^2 I - p p^T)   (Gram-Schmidt of these fields, as the
+    // reference does, equals multiplication by the Cholesky inverse, so the
+    // projector sum_i n_i n_i^T reduces to G^{-1} in mode coordinates)
+    double G[3][3] = {{0}};
+    const int n = m.n, nr = m.nr;
+    for (int64_t lat = 0; lat < 10LL * n * n; ++lat) {
+        if (!m.owned_lat[lat]) continue;
+        const double* c = &m.coords[3 * lat];
+        for (int r = 0; r < nr; ++r) {
+            const double p[3] = {c[0] * m.radii[r], c[1] * m.radii[r], c[2] * m.radii[r]};
+            const double pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) G[a][b] += (a == b ? pp : 0.0) - p[a] * p[b];
+        }
+    }
+    const double det = G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1]) -
+                       G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0]) +
+                       G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
+    double S[9];
+    S[0] = (G[1][1] * G[2][2] - G[1][2] * G[2][1]) / det;
+    S[1] = (G[0][2] * G[2][1] - G[0][1] * G[2][2]) / det;
+    S[2] = (G[0][1] * G[1][2] - G[0][2] * G[1][1]) / det;
+    S[3] = (G[1][2] * G[2][0] - G[1][0] * G[2][2]) / det;
+    S[4] = (G[0][0] * G[2][2] - G[0][2] * G[2][0]) / det;
+    S[5] = (G[0][2] * G[1][0] - G[0][0] * G[1][2]) / det;
+    S[6] = (G[1][0] * G[2][1] - G[1][1] * G[2][0]) / det;
+    S[7] = (G[0][1] * G[2][0] - G[0][0] * G[2][1]) / det;
+    S[8] = (G[0][0] * G[1][1] - G[0][1] * G[1][0]) / det;
+    std::vector<double> h(16, 0.0);
+    for (int i = 0; i < 9; ++i) h[4 + i] = op->penalty_eps * S[i];
+    op->pen.upload(h);
+    op->penalty_ready = true;
+}
+
+template <typename T>
+tn::Geom<T> geom_of(const MeshLevel& m);
+template <>
+tn::Geom<double> geom_of<double>(const MeshLevel& m)
+{
+    return {m.d_coords.p, m.d_radii.p, m.d_geo.p, m.n, m.nr};
+}
+template <>
+tn::Geom<float> geom_of<float>(const MeshLevel& m)
+{
+    return {m.d_coords_f.p, m.d_radii_f.p, m.d_geo_f.p, m.n, m.nr};
+}
+
+template <typename T>
+void eps_apply_t(tn_eps* op, const void* src_v, void* dst_v, hipStream_t s)
+{
+    const MeshLevel& m = const_cast<tn_mesh*>(op->mesh)->lvl.dev();
+    const T* src = static_cast<const T*>(src_v);
+    T* dst = static_cast<T*>(dst_v);
+    const T* eta = static_cast<const T*>(op->eta);
+    TN_REQUIRE(src && dst && eta, "eps apply: null pointer");
+    TN_REQUIRE((const void*)src != (void*)dst, "eps apply: src and dst must differ");
+    const bool penalty = !op->diagonal && op->bc_cmb == TN_FREESLIP && op->bc_surf == TN_FREESLIP;
+    if (penalty) {
+        if (!op->penalty_ready) build_penalty(op);
+        tn::launch_fill<double>(op->pen.p, 4, 0.0, s);
+        tn::launch_penalty_moment<T>(m, src, op->pen.p, s);
+        tn::launch_penalty_init<T>(m, op->pen.p, op->pen.p + 4, dst, op->apply_mode == TN_ADD, s);
+    } else if (op->apply_mode == TN_REPLACE) {
+        tn::launch_fill<T>(dst, 3 * m.N, T(0), s);
+    }
+    tn::EpsLaunch cfg;
+    cfg.bc_cmb = op->bc_cmb;
+    cfg.bc_surf = op->bc_surf;
+    cfg.diagonal = op->diagonal;
+    tn::launch_eps_apply<T>(geom_of<T>(m), eta, src, dst, cfg, s);
+    if (op->comm_mode == TN_COMM_ADDITIVE) tn::launch_interface_sum<T>(m, 3, dst, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tn_last_error(void) { return tn::g_last_error.c_str(); }
+int tn_version(void) { return 1; }
+
+int tn_malloc(void** ptr, size_t bytes)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(ptr, "null out pointer");
+    TN_HIP(hipMalloc(ptr, bytes));
+    TN_API_END
+}
+int tn_free(void* ptr)
+{
+    TN_API_BEGIN
+    TN_HIP(hipFree(ptr));
+    TN_API_END
+}
+int tn_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    TN_API_BEGIN
+    TN_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    TN_API_END
+}
+int tn_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    TN_API_BEGIN
+    TN_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    TN_API_END
+}
+int tn_sync(void)
+{
+    TN_API_BEGIN
+    TN_HIP(hipDeviceSynchronize());
+    TN_API_END
+}
+
+// ------------------------------------------------------------------ mesh
+static tn_mesh* new_mesh(int lat_level, const std::vector<double>& radii)
+{
+    auto m = std::make_unique<tn_mesh>();
+    m->lvl.build(lat_level, radii);
+    return m.release();
+}
+
+int tn_mesh_create(int lat_level, int rad_level, double r_min, double r_max, tn_mesh** out)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(out, "null out pointer");
+    TN_REQUIRE(rad_level >= 1 && rad_level <= 12, "radial level out of range [1,12]");
+    TN_REQUIRE(r_min < r_max, "r_min must be < r_max");
+    const int nr = (1 << rad_level) + 1;
+    std::vector<double> radii(nr);
+    // uniform_shell_radii, spherical_shell.hpp:27-52
+    const double step = (r_max - r_min) / (nr - 1);
+    for (int i = 0; i < nr; ++i) radii[i] = r_min + i * step;
+    radii[nr - 1] = r_max;
+    *out = new_mesh(lat_level, radii);
+    TN_API_END
+}
+
+int tn_mesh_create_radii(int lat_level, int nr, const double* radii, tn_mesh** out)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(out && radii, "null pointer");
+    *out = new_mesh(lat_level, std::vector<double>(radii, radii + nr));
+    TN_API_END
+}
+
+int tn_mesh_destroy(tn_mesh* m)
+{
+    TN_API_BEGIN
+    delete m;
+    TN_API_END
+}
+
+int tn_mesh_dims(const tn_mesh* m, int* n, int* nr, int64_t* num_nodes)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m, "null mesh");
+    if (n) *n = m->lvl.n;
+    if (nr) *nr = m->lvl.nr;
+    if (num_nodes) *num_nodes = m->lvl.N;
+    TN_API_END
+}
+
+int64_t tn_mesh_owned_nodes(const tn_mesh* m) { return m ? m->lvl.owned : -1; }
+
+int tn_mesh_lateral_coords(const tn_mesh* m, double* out)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && out, "null pointer");
+    std::memcpy(out, m->lvl.coords.data(), sizeof(double) * m->lvl.coords.size());
+    TN_API_END
+}
+
+int tn_mesh_radii(const tn_mesh* m, double* out)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && out, "null pointer");
+    std::memcpy(out, m->lvl.radii.data(), sizeof(double) * m->lvl.radii.size());
+    TN_API_END
+}
+
+int tn_mesh_owned_mask(const tn_mesh* m, uint8_t* out)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && out, "null pointer");
+    const auto& L = m->lvl;
+    for (int64_t lat = 0; lat < 10LL * L.n * L.n; ++lat)
+        for (int r = 0; r < L.nr; ++r) out[lat * L.nr + r] = L.owned_lat[lat];
+    TN_API_END
+}
+
+int tn_mesh_interface_info(const tn_mesh* m, int* ngroups, int* nmembers)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m, "null mesh");
+    if (ngroups) *ngroups = m->lvl.ngroups();
+    if (nmembers) *nmembers = (int)m->lvl.grp_mem.size();
+    TN_API_END
+}
+
+// ------------------------------------------------------------------ BLAS-1
+int tn_interface_sum(const tn_mesh* m, int dtype, int ncomp, void* v, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && v, "null pointer");
+    TN_REQUIRE(ncomp == 1 || ncomp == 3, "ncomp must be 1 or 3");
+    const MeshLevel& L = const_cast<tn_mesh*>(m)->lvl.dev();
+    if (dtype == TN_F64)
+        tn::launch_interface_sum<double>(L, ncomp, (double*)v, (hipStream_t)s);
+    else
+        tn::launch_interface_sum<float>(L, ncomp, (float*)v, (hipStream_t)s);
+    TN_API_END
+}
+
+int tn_lincomb(const tn_mesh* m, int dtype, int ncomp, void* y, int nx, const double* c, const void* const* x,
+               double c0, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && y, "null pointer");
+    TN_REQUIRE(nx == 0 || (c && x), "null coefficient/input arrays");
+    const int64_t cnt = m->lvl.N * ncomp;
+    if (dtype == TN_F64)
+        tn::launch_lincomb<double>(cnt, (double*)y, nx, c, (const double* const*)x, c0, (hipStream_t)s);
+    else
+        tn::launch_lincomb<float>(cnt, (float*)y, nx, c, (const float* const*)x, c0, (hipStream_t)s);
+    TN_API_END
+}
+
+int tn_dot(const tn_mesh* m, int dtype, int ncomp, const void* a, const void* b, double* out, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && a && b && out, "null pointer");
+    auto& L = const_cast<tn_mesh*>(m)->lvl.dev();
+    tn::launch_fill<double>(L.d_scratch.p, 1, 0.0, (hipStream_t)s);
+    if (dtype == TN_F64)
+        tn::launch_dot_owned<double>(L, ncomp, (const double*)a, (const double*)b, L.d_scratch.p, (hipStream_t)s);
+    else
+        tn::launch_dot_owned<float>(L, ncomp, (const float*)a, (const float*)b, L.d_scratch.p, (hipStream_t)s);
+    TN_HIP(hipMemcpyAsync(out, L.d_scratch.p, sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)s));
+    TN_HIP(hipStreamSynchronize((hipStream_t)s));
+    TN_API_END
+}
+
+// ------------------------------------------------------------------ epsilon operator
+int tn_eps_create(const tn_mesh* m, int dtype, const void* eta, int bc_cmb, int bc_surf, tn_eps** out)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && out, "null pointer");
+    TN_REQUIRE(dtype == TN_F64 || dtype == TN_F32, "dtype must be TN_F64 or TN_F32");
+    TN_REQUIRE(bc_cmb >= 0 && bc_cmb <= 2 && bc_surf >= 0 && bc_surf <= 2, "invalid boundary condition");
+    auto op = std::make_unique<tn_eps>();
+    op->mesh = m;
+    op->dtype = dtype;
+    op->eta = eta;
+    op->bc_cmb = bc_cmb;
+    op->bc_surf = bc_surf;
+    *out = op.release();
+    TN_API_END
+}
+
+int tn_eps_destroy(tn_eps* op)
+{
+    TN_API_BEGIN
+    delete op;
+    TN_API_END
+}
+
+int tn_eps_set_modes(tn_eps* op, int apply_mode, int comm_mode)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op, "null operator");
+    TN_REQUIRE(apply_mode == TN_REPLACE || apply_mode == TN_ADD, "invalid apply mode");
+    TN_REQUIRE(comm_mode == TN_COMM_SKIP || comm_mode == TN_COMM_ADDITIVE, "invalid communication mode");
+    op->apply_mode = apply_mode;
+    op->comm_mode = comm_mode;
+    TN_API_END
+}
+
+int tn_eps_set_diagonal(tn_eps* op, int diagonal)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op, "null operator");
+    op->diagonal = diagonal != 0;
+    TN_API_END
+}
+
+int tn_eps_set_boundary_conditions(tn_eps* op, int bc_cmb, int bc_surf)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op, "null operator");
+    TN_REQUIRE(bc_cmb >= 0 && bc_cmb <= 2 && bc_surf >= 0 && bc_surf <= 2, "invalid boundary condition");
+    op->bc_cmb = bc_cmb;
+    op->bc_surf = bc_surf;
+    TN_API_END
+}
+
+int tn_eps_set_eta(tn_eps* op, const void* eta)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op, "null operator");
+    op->eta = eta;
+    TN_API_END
+}
+
+int tn_eps_apply(tn_eps* op, const void* src, void* dst, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op, "null operator");
+    if (op->dtype == TN_F64)
+        eps_apply_t<double>(op, src, dst, (hipStream_t)s);
+    else
+        eps_apply_t<float>(op, src, dst, (hipStream_t)s);
+    TN_API_END
+}
+
+}  // extern "C"

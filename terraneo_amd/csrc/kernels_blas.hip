@@ -1,0 +1,235 @@
+// Level-1 kernels on mesh fields: fill, linear combinations, owned-node dots,
+// the additive interface sum over duplicated diamond-edge nodes, and the
+// free-slip rotation-penalty helpers.
+//
+// Reference counterparts: kernels/common/grid_operations.hpp:44-785
+// (set/lincomb/masked_dot_product), communication/shell/communication.hpp:241-739
+// (additive interface reduction, single-process form), epsilon_divdiv_kerngen.hpp:287-408,583-607
+// (fs/fs penalty).
+#include "tn_internal.hpp"
+
+namespace tn {
+namespace {
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ y, int64_t count, T v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = v;
+}
+
+template <typename T>
+struct LinArgs {
+    const T* x[3];
+    double c[3];
+    double c0;
+    int nx;
+};
+
+template <typename T, int NX>
+__global__ void k_lincomb(T* __restrict__ y, int64_t count, LinArgs<T> a)
+{
+    const T c0 = (T)a.c0;
+    T c[3] = {(T)a.c[0], (T)a.c[1], (T)a.c[2]};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        T v = c0;
+        if (NX > 0) v += c[0] * a.x[0][i];
+        if (NX > 1) v += c[1] * a.x[1][i];
+        if (NX > 2) v += c[2] * a.x[2][i];
+        y[i] = v;
+    }
+}
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block-reduce `v` into out[slot] (double atomics into device memory).
+__device__ inline void block_accumulate(double v, double* out)
+{
+    __shared__ double part[16];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) part[wv] = v;
+    __syncthreads();
+    if (wv == 0) {
+        const int nw = blockDim.x >> 6;
+        double s = lane < nw ? part[lane] : 0.0;
+        s = wave_sum(s);
+        if (lane == 0) unsafeAtomicAdd(out, s);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot_owned(const T* __restrict__ a, const T* __restrict__ b,
+                                                   const uint8_t* __restrict__ owned_lat, int64_t N, int nr,
+                                                   int ncomp, double* out)
+{
+    double s = 0.0;
+    const int64_t total = N * ncomp;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t node = i % N;
+        if (owned_lat[node / nr]) s += (double)a[i] * (double)b[i];
+    }
+    block_accumulate(s, out);
+}
+
+// One thread per (duplicated lateral group, radial index), r fastest.
+template <typename T>
+__global__ void k_interface_sum(const int32_t* __restrict__ off, const int32_t* __restrict__ mem, int ngroups,
+                                int nr, int64_t N, int ncomp, T* __restrict__ v)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)ngroups * nr) return;
+    const int g = (int)(tid / nr), r = (int)(tid % nr);
+    const int b = off[g], e = off[g + 1];
+    for (int c = 0; c < ncomp; ++c) {
+        T* vc = v + c * N;
+        double s = 0.0;
+        for (int k = b; k < e; ++k) s += (double)vc[(int64_t)mem[k] * nr + r];
+        for (int k = b; k < e; ++k) vc[(int64_t)mem[k] * nr + r] = (T)s;
+    }
+}
+
+// beta += sum over owned nodes of (x_node) cross u_node
+template <typename T>
+__global__ __launch_bounds__(256) void k_penalty_moment(const T* __restrict__ u, const double* __restrict__ coords,
+                                                        const double* __restrict__ radii,
+                                                        const uint8_t* __restrict__ owned_lat, int64_t N, int nr,
+                                                        double* beta)
+{
+    double m0 = 0, m1 = 0, m2 = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t lat = i / nr;
+        if (!owned_lat[lat]) continue;
+        const double r = radii[i % nr];
+        const double px = coords[3 * lat] * r, py = coords[3 * lat + 1] * r, pz = coords[3 * lat + 2] * r;
+        const double ux = u[i], uy = u[N + i], uz = u[2 * N + i];
+        m0 += py * uz - pz * uy;
+        m1 += pz * ux - px * uz;
+        m2 += px * uy - py * ux;
+    }
+    block_accumulate(m0, beta);
+    __syncthreads();
+    block_accumulate(m1, beta + 1);
+    __syncthreads();
+    block_accumulate(m2, beta + 2);
+}
+
+// y (+)= gamma x p, gamma = S beta (S: 3x3 row-major)
+template <typename T>
+__global__ void k_penalty_init(const double* __restrict__ beta, const double* __restrict__ S,
+                               const double* __restrict__ coords, const double* __restrict__ radii, int64_t N, int nr,
+                               T* __restrict__ y, int add)
+{
+    const double g0 = S[0] * beta[0] + S[1] * beta[1] + S[2] * beta[2];
+    const double g1 = S[3] * beta[0] + S[4] * beta[1] + S[5] * beta[2];
+    const double g2 = S[6] * beta[0] + S[7] * beta[1] + S[8] * beta[2];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t lat = i / nr;
+        const double r = radii[i % nr];
+        const double px = coords[3 * lat] * r, py = coords[3 * lat + 1] * r, pz = coords[3 * lat + 2] * r;
+        const T vx = (T)(g1 * pz - g2 * py), vy = (T)(g2 * px - g0 * pz), vz = (T)(g0 * py - g1 * px);
+        if (add) {
+            y[i] += vx;
+            y[N + i] += vy;
+            y[2 * N + i] += vz;
+        } else {
+            y[i] = vx;
+            y[N + i] = vy;
+            y[2 * N + i] = vz;
+        }
+    }
+}
+
+inline unsigned grid_for(int64_t count, int block = 256, int64_t cap = 16384)
+{
+    int64_t g = (count + block - 1) / block;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+template <typename T>
+void launch_fill(T* y, int64_t count, T value, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(count)), dim3(256), 0, s, y, count, value);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_lincomb(int64_t count, T* y, int nx, const double* c, const T* const* x, double c0, hipStream_t s)
+{
+    TN_REQUIRE(nx >= 0 && nx <= 3, "lincomb supports at most 3 inputs");
+    LinArgs<T> a{};
+    a.nx = nx;
+    a.c0 = c0;
+    for (int i = 0; i < nx; ++i) {
+        a.x[i] = x[i];
+        a.c[i] = c[i];
+    }
+    const dim3 gr(grid_for(count)), bl(256);
+    switch (nx) {
+    case 0: hipLaunchKernelGGL((k_lincomb<T, 0>), gr, bl, 0, s, y, count, a); break;
+    case 1: hipLaunchKernelGGL((k_lincomb<T, 1>), gr, bl, 0, s, y, count, a); break;
+    case 2: hipLaunchKernelGGL((k_lincomb<T, 2>), gr, bl, 0, s, y, count, a); break;
+    default: hipLaunchKernelGGL((k_lincomb<T, 3>), gr, bl, 0, s, y, count, a); break;
+    }
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, double* out_dev, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dot_owned<T>, dim3(grid_for(m.N * ncomp, 256, 2048)), dim3(256), 0, s, a, b,
+                       m.d_owned_lat.p, m.N, m.nr, ncomp, out_dev);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_interface_sum(const MeshLevel& m, int ncomp, T* v, hipStream_t s)
+{
+    const int ng = m.ngroups();
+    if (ng <= 0) return;
+    const int64_t total = (int64_t)ng * m.nr;
+    hipLaunchKernelGGL(k_interface_sum<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, m.d_grp_off.p,
+                       m.d_grp_mem.p, ng, m.nr, m.N, ncomp, v);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_penalty_moment(const MeshLevel& m, const T* u, double* beta_dev, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_penalty_moment<T>, dim3(grid_for(m.N, 256, 2048)), dim3(256), 0, s, u, m.d_coords.p,
+                       m.d_radii.p, m.d_owned_lat.p, m.N, m.nr, beta_dev);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_penalty_init(const MeshLevel& m, const double* beta_dev, const double* S_dev, T* y, bool add,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_penalty_init<T>, dim3(grid_for(m.N)), dim3(256), 0, s, beta_dev, S_dev, m.d_coords.p,
+                       m.d_radii.p, m.N, m.nr, y, (int)add);
+    TN_HIP(hipGetLastError());
+}
+
+#define TN_INST(T)                                                                                               \
+    template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                                   \
+    template void launch_lincomb<T>(int64_t, T*, int, const double*, const T* const*, double, hipStream_t);     \
+    template void launch_dot_owned<T>(const MeshLevel&, int, const T*, const T*, double*, hipStream_t);          \
+    template void launch_interface_sum<T>(const MeshLevel&, int, T*, hipStream_t);                               \
+    template void launch_penalty_moment<T>(const MeshLevel&, const T*, double*, hipStream_t);                    \
+    template void launch_penalty_init<T>(const MeshLevel&, const double*, const double*, T*, bool, hipStream_t);
+TN_INST(double)
+TN_INST(float)
+#undef TN_INST
+
+}  // namespace tn

@@ -1,0 +1,148 @@
+// Internal declarations shared by the host C++ and the HIP kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/terraneo_amd.h"
+
+namespace tn {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void set_last_error(const std::string& msg);
+
+#define TN_HIP(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            throw ::tn::Error(std::string("HIP error ") + hipGetErrorString(e_) + " at " __FILE__ ":" + \
+                              std::to_string(__LINE__) + " in " #call);                               \
+    } while (0)
+
+#define TN_REQUIRE(cond, msg)                              \
+    do {                                                   \
+        if (!(cond)) throw ::tn::Error(std::string(msg)); \
+    } while (0)
+
+// ---------------------------------------------------------------- device buffers
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void alloc(size_t count)
+    {
+        release();
+        n = count;
+        if (count) TN_HIP(hipMalloc(&p, sizeof(T) * count));
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const std::vector<T>& h)
+    {
+        alloc(h.size());
+        if (n) TN_HIP(hipMemcpy(p, h.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+    }
+};
+
+// ---------------------------------------------------------------- mesh level
+// Lateral geometry of one wedge, radius-independent part of J^{-T}:
+//   J = A diag(r_mid, r_mid, dr/2),  A = [v1-v0, v2-v0, (v0+v1+v2)/3]
+//   grad N (lateral vertex i, bottom/top) = l_i / r_mid  -/+  z * 2/dr
+//   l1 = A^{-T}e0 / 2, l2 = A^{-T}e1 / 2, l0 = -(l1+l2), z = A^{-T}e2 / 6
+//   |det J| = r_mid^2 (dr/2) |det A|
+constexpr int GEO_STRIDE = 10;  // l1[3] l2[3] z[3] |detA|
+
+struct MeshLevel {
+    int lat_level = 0;
+    int n = 0, nr = 0;
+    int64_t N = 0;        // stored nodes
+    int64_t owned = 0;    // owned nodes
+    std::vector<double> coords;     // [10][n][n][3]
+    std::vector<double> radii;      // [nr]
+    std::vector<uint8_t> owned_lat; // [10][n][n]
+    std::vector<int32_t> grp_off;   // CSR over duplicated lateral nodes (>= 2 copies)
+    std::vector<int32_t> grp_mem;   // lateral index d*n*n + x*n + y
+    std::vector<double> geo;        // [10][n-1][n-1][2][GEO_STRIDE]
+
+    DevBuf<double> d_coords;
+    DevBuf<float> d_coords_f;
+    DevBuf<double> d_radii;
+    DevBuf<float> d_radii_f;
+    DevBuf<double> d_geo;
+    DevBuf<float> d_geo_f;
+    DevBuf<uint8_t> d_owned_lat;
+    DevBuf<int32_t> d_grp_off;
+    DevBuf<int32_t> d_grp_mem;
+    DevBuf<double> d_scratch;  // reduction scratch (doubles)
+    bool uploaded = false;
+
+    int ngroups() const { return (int)grp_off.size() - 1; }
+    void build(int lat_level, const std::vector<double>& radii_in);
+    void upload();
+    // device copies are made on first use, so host-only queries work without a GPU
+    MeshLevel& dev()
+    {
+        if (!uploaded) upload();
+        return *this;
+    }
+};
+
+// ---------------------------------------------------------------- kernel entry points (HIP files)
+template <typename T>
+struct Geom {
+    const T* coords;     // [10][n][n][3]
+    const T* radii;      // [nr]
+    const T* geo;        // [10][n-1][n-1][2][GEO_STRIDE]
+    int n, nr;
+};
+
+struct EpsLaunch {
+    int bc_cmb = TN_NEUMANN, bc_surf = TN_NEUMANN;
+    bool diagonal = false;
+};
+
+template <typename T>
+void launch_eps_apply(const Geom<T>& g, const T* eta, const T* src, T* dst, const EpsLaunch& cfg, hipStream_t s);
+
+template <typename T>
+void launch_fill(T* y, int64_t count, T value, hipStream_t s);
+
+template <typename T>
+void launch_interface_sum(const MeshLevel& m, int ncomp, T* v, hipStream_t s);
+
+template <typename T>
+void launch_lincomb(int64_t count, T* y, int nx, const double* c, const T* const* x, double c0, hipStream_t s);
+
+// out_dev[0..nd) += sum over owned nodes, accumulate in double; ncomp fields of N each
+template <typename T>
+void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, double* out_dev, hipStream_t s);
+
+// fs/fs penalty: beta[0..3) = sum_owned r x u  ;  y (+)= gamma x r with gamma = S beta
+template <typename T>
+void launch_penalty_moment(const MeshLevel& m, const T* u, double* beta_dev, hipStream_t s);
+template <typename T>
+void launch_penalty_init(const MeshLevel& m, const double* beta_dev, const double* S_dev, T* y, bool add,
+                         hipStream_t s);
+
+}  // namespace tn
+
+struct tn_mesh {
+    tn::MeshLevel lvl;
+};
