@@ -41,6 +41,42 @@ enum tn_comm_mode { TN_COMM_SKIP = 0, TN_COMM_ADDITIVE = 1 };
 
 typedef struct tn_mesh tn_mesh;
 typedef struct tn_eps tn_eps;
+typedef struct tn_stokes tn_stokes;
+typedef struct tn_gmg tn_gmg;
+
+/* Stokes solver variants (block preconditioners of block_preconditioner_2x2.hpp) */
+enum tn_stokes_method { TN_MINRES_BLOCKDIAG = 0, TN_FGMRES_BLOCKTRI = 1, TN_FGMRES_BLOCKDIAG = 2 };
+
+/* geometric multigrid options (Chebyshev parameters chebyshev.hpp:50-80,
+ * Multigrid cycles multigrid.hpp:90-104, coarsest PCG pcg.hpp:45-66) */
+typedef struct tn_gmg_options {
+    int cheby_order;           /* Chebyshev degree                           */
+    int cheby_iterations;      /* smoothing steps per pre/post smoothing     */
+    int power_iterations;      /* lambda_max(D^-1 A) estimate                */
+    int num_cycles;            /* V-cycles per application                   */
+    int coarse_max_iterations; /* coarsest-level PCG                         */
+    double coarse_rtol;
+    double coarse_atol;
+    uint64_t seed;             /* power-iteration start vector               */
+} tn_gmg_options;
+
+/* IterativeSolverParameters / FGMRESOptions (iterative_solver_info.hpp, fgmres.hpp:20-44) */
+typedef struct tn_krylov_options {
+    int max_iterations;
+    double rtol;
+    double atol;
+    int restart; /* FGMRES only */
+} tn_krylov_options;
+
+typedef struct tn_solve_info {
+    int iterations;
+    double initial_residual;
+    double final_residual;
+    double seconds;
+    int history_len;  /* number of residual entries produced (iteration 0 included) */
+    int restarts;     /* FGMRES breakdown / stagnation restarts */
+    int prec_skipped; /* FGMRES preconditioner outputs rejected as non-finite */
+} tn_solve_info;
 
 /* ------------------------------------------------------------------ misc */
 const char* tn_last_error(void);
@@ -103,9 +139,73 @@ int tn_eps_set_modes(tn_eps* op, int apply_mode, int comm_mode);
 int tn_eps_set_diagonal(tn_eps* op, int diagonal);
 /* set_boundary_conditions, kerngen.hpp:420-427 */
 int tn_eps_set_boundary_conditions(tn_eps* op, int bc_cmb, int bc_surf);
+/* Replace the viscosity buffer (the reference operator's k_ view, kerngen.hpp:94-96).
+ * The operator keeps a pointer, not a copy, but caches per-viscosity data for
+ * the two boundary shells: after changing viscosity values in place, call this
+ * again (the same pointer is fine) before the next apply. */
 int tn_eps_set_eta(tn_eps* op, const void* eta);
 /* apply_impl, kerngen.hpp:513-614 */
 int tn_eps_apply(tn_eps* op, const void* src, void* dst, tn_stream_t s);
+/* penalty epsilon of the fs/fs rotation penalty (set_penalty_epsilon, kerngen.hpp:279) */
+int tn_eps_set_penalty_epsilon(tn_eps* op, double eps);
+/* Galerkin coarse operator from the next finer operator (TwoGridGCA, gca.hpp:72-531;
+ * switches `coarse` to its stored-matrix path, set_stored_matrix_mode kerngen.hpp:443-464).
+ * Matrices are stored symmetric-packed in TN_F64 or TN_F32. */
+int tn_eps_gca_from(tn_eps* coarse, const tn_eps* fine, int store_dtype, tn_stream_t s);
+/* get_local_matrix of a stored operator (kerngen.hpp:481-495), host out[18*18],
+ * dimension-wise dof order (component*6 + wedge node); synchronous */
+int tn_eps_get_local_matrix(const tn_eps* op, int sd, int x, int y, int r, int wedge, double* out);
+
+/* ------------------------------------------------------------------ Stokes block
+ * EpsDivDivStokes (epsilon_divdiv_stokes.hpp:19-130): [A B^T; B 0] with the
+ * Q1-iso-Q2 velocity on `vel` and Q1 pressure on `pre` (one level coarser).
+ * Pressure fields are T[10][n_p][n_p][nr_p]. */
+int tn_stokes_create(const tn_mesh* vel, const tn_mesh* pre, tn_eps* A, tn_stokes** out);
+int tn_stokes_destroy(tn_stokes* K);
+int tn_stokes_apply(tn_stokes* K, const void* u, const void* p, void* yu, void* yp, tn_stream_t s);
+/* Divergence B (divergence.hpp:90-115): p = B u, replace + additive communication */
+int tn_div_apply(tn_stokes* K, const void* u, void* p, tn_stream_t s);
+/* Gradient B^T (gradient.hpp:87-112) */
+int tn_grad_apply(tn_stokes* K, const void* p, void* u, int apply_mode, int comm_mode, tn_stream_t s);
+
+/* ------------------------------------------------------------------ mass, transfers, boundary helpers */
+/* VectorMass (vector_mass.hpp:59-153): y = M x, replace + communication */
+int tn_vector_mass_apply(const tn_mesh* m, int dtype, const void* x, void* y, tn_stream_t s);
+/* lumped KMass (kmass.hpp:17-185, set_lumped_diagonal): d = rowsum(M_k) */
+int tn_kmass_lumped(const tn_mesh* m, int dtype, const void* k, void* d, tn_stream_t s);
+/* ProlongationVecConstant in Add mode (prolongation_constant.hpp:103-193): xf += P xc */
+int tn_prolongate_add(const tn_mesh* coarse, const tn_mesh* fine, int dtype, int ncomp, const void* xc, void* xf,
+                      tn_stream_t s);
+/* RestrictionVecConstant (restriction_constant.hpp:134-250): xc = R xf */
+int tn_restrict(const tn_mesh* fine, const tn_mesh* coarse, int dtype, int ncomp, const void* xf, void* xc,
+                tn_stream_t s);
+/* assign_masked_else_keep_old on shells: which = 1 (CMB) | 2 (SURFACE);
+ * src NULL assigns zero (strong_algebraic_dirichlet_enforcement.hpp:88-123) */
+int tn_boundary_assign(const tn_mesh* m, int dtype, int ncomp, void* dst, const void* src, int which, tn_stream_t s);
+/* strong_algebraic_freeslip_enforcement_in_place (strong_algebraic_freeslip_enforcement.hpp:21-37) */
+int tn_freeslip_project(const tn_mesh* m, int dtype, void* v, int which, tn_stream_t s);
+
+/* ------------------------------------------------------------------ solvers
+ * Multigrid with Chebyshev smoothing over `nlevels` operators ordered coarse
+ * to fine (Multigrid multigrid.hpp:24-212, Chebyshev chebyshev.hpp:50-177).
+ * The operators are referenced, not owned. */
+int tn_gmg_create(int nlevels, tn_eps* const* ops, const tn_gmg_options* opts, tn_stream_t s, tn_gmg** out);
+int tn_gmg_destroy(tn_gmg* g);
+/* x <- V-cycle(s) for A_finest x = b, starting from x */
+int tn_gmg_solve(tn_gmg* g, void* x, const void* b, tn_stream_t s);
+int tn_gmg_get_lambda(const tn_gmg* g, int level, double* lambda);
+int tn_gmg_set_lambda(tn_gmg* g, int level, double lambda);
+/* PCG (pcg.hpp:78-153) on the viscous operator, optional multigrid
+ * preconditioner; history (if not NULL) receives up to history_cap residuals */
+int tn_pcg_solve(tn_eps* A, tn_gmg* prec, void* x, const void* b, const tn_krylov_options* o, tn_solve_info* info,
+                 double* history, int history_cap, tn_stream_t s);
+/* Stokes solve: PMINRES with block-diagonal preconditioner or FGMRES(m) with
+ * block-triangular / block-diagonal preconditioner; velocity block by
+ * `prec_u` (its dtype may differ: FP32 multigrid inside FP64 Krylov), Schur
+ * block by the inverse of `schur_diag` (pressure field, e.g. lumped M_p(1/eta)) */
+int tn_stokes_solve(tn_stokes* K, int method, tn_gmg* prec_u, const void* schur_diag, void* u, void* p,
+                    const void* bu, const void* bp, const tn_krylov_options* o, tn_solve_info* info, double* history,
+                    int history_cap, tn_stream_t s);
 
 #ifdef __cplusplus
 }

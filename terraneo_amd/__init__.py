@@ -8,6 +8,12 @@ and exports its symbols); compute calls need one.
 """
 from ._capi import (  # noqa: F401
     LIB_PATH,
+    TN_FGMRES_BLOCKDIAG,
+    TN_FGMRES_BLOCKTRI,
+    TN_MINRES_BLOCKDIAG,
+    GmgOptions,
+    KrylovOptions,
+    SolveInfo,
     TN_ADD,
     TN_COMM_ADDITIVE,
     TN_COMM_SKIP,
@@ -20,4 +26,5 @@ from ._capi import (  # noqa: F401
     TNError,
     lib,
 )
-from .api import Eps, Mesh  # noqa: F401
+from .api import Eps, Gmg, Mesh, Stokes, pcg_solve, stokes_solve  # noqa: F401
+from . import api  # noqa: F401

@@ -12,6 +12,23 @@ TN_F64, TN_F32 = 0, 1
 TN_NEUMANN, TN_DIRICHLET, TN_FREESLIP = 0, 1, 2
 TN_REPLACE, TN_ADD = 0, 1
 TN_COMM_SKIP, TN_COMM_ADDITIVE = 0, 1
+TN_MINRES_BLOCKDIAG, TN_FGMRES_BLOCKTRI, TN_FGMRES_BLOCKDIAG = 0, 1, 2
+
+
+class GmgOptions(C.Structure):
+    _fields_ = [("cheby_order", C.c_int), ("cheby_iterations", C.c_int), ("power_iterations", C.c_int),
+                ("num_cycles", C.c_int), ("coarse_max_iterations", C.c_int), ("coarse_rtol", C.c_double),
+                ("coarse_atol", C.c_double), ("seed", C.c_uint64)]
+
+
+class KrylovOptions(C.Structure):
+    _fields_ = [("max_iterations", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("restart", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("initial_residual", C.c_double), ("final_residual", C.c_double),
+                ("seconds", C.c_double), ("history_len", C.c_int), ("restarts", C.c_int),
+                ("prec_skipped", C.c_int)]
 
 
 class TNError(RuntimeError):
@@ -52,6 +69,27 @@ SIGNATURES = {
     "tn_eps_set_boundary_conditions": (i32, [vp, i32, i32]),
     "tn_eps_set_eta": (i32, [vp, vp]),
     "tn_eps_apply": (i32, [vp, vp, vp, vp]),
+    "tn_eps_set_penalty_epsilon": (i32, [vp, f64]),
+    "tn_eps_gca_from": (i32, [vp, vp, i32, vp]),
+    "tn_eps_get_local_matrix": (i32, [vp, i32, i32, i32, i32, i32, vp]),
+    "tn_stokes_create": (i32, [vp, vp, vp, pvp]),
+    "tn_stokes_destroy": (i32, [vp]),
+    "tn_stokes_apply": (i32, [vp, vp, vp, vp, vp, vp]),
+    "tn_div_apply": (i32, [vp, vp, vp, vp]),
+    "tn_grad_apply": (i32, [vp, vp, vp, i32, i32, vp]),
+    "tn_vector_mass_apply": (i32, [vp, i32, vp, vp, vp]),
+    "tn_kmass_lumped": (i32, [vp, i32, vp, vp, vp]),
+    "tn_prolongate_add": (i32, [vp, vp, i32, i32, vp, vp, vp]),
+    "tn_restrict": (i32, [vp, vp, i32, i32, vp, vp, vp]),
+    "tn_boundary_assign": (i32, [vp, i32, i32, vp, vp, i32, vp]),
+    "tn_freeslip_project": (i32, [vp, i32, vp, i32, vp]),
+    "tn_gmg_create": (i32, [i32, pvp, vp, vp, pvp]),
+    "tn_gmg_destroy": (i32, [vp]),
+    "tn_gmg_solve": (i32, [vp, vp, vp, vp]),
+    "tn_gmg_get_lambda": (i32, [vp, i32, C.POINTER(f64)]),
+    "tn_gmg_set_lambda": (i32, [vp, i32, f64]),
+    "tn_pcg_solve": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, vp]),
+    "tn_stokes_solve": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp]),
 }
 
 

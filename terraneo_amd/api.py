@@ -8,7 +8,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._capi import TN_COMM_ADDITIVE, TN_F32, TN_F64, TN_NEUMANN, TN_REPLACE, check, lib
+from ._capi import (TN_ADD, TN_COMM_ADDITIVE, TN_F32, TN_F64, TN_NEUMANN, TN_REPLACE, GmgOptions, KrylovOptions,
+                    SolveInfo, check, lib)
 
 
 def _stream_ptr(stream=None):
@@ -137,3 +138,141 @@ class Eps:
                 self.h = None
         except Exception:
             pass
+
+    # ---- Galerkin coarse operators
+    def gca_from(self, fine, store_dtype=TN_F64, stream=None):
+        """Make this operator's stored matrices the Galerkin coarsening of `fine`."""
+        self.fine_ref = fine
+        check(lib().tn_eps_gca_from(self.h, fine.h, store_dtype, _stream_ptr(stream)))
+
+    def local_matrix(self, sd, x, y, r, w):
+        out = np.zeros(324)
+        check(lib().tn_eps_get_local_matrix(self.h, sd, x, y, r, w, out.ctypes.data))
+        return out.reshape(18, 18)
+
+    def set_penalty_epsilon(self, eps):
+        check(lib().tn_eps_set_penalty_epsilon(self.h, eps))
+
+
+# ---------------------------------------------------------------- mesh-level helpers
+def vector_mass(mesh, x, y, stream=None):
+    check(lib().tn_vector_mass_apply(mesh.h, dtype_code(x.dtype), _ptr(x), _ptr(y), _stream_ptr(stream)))
+    return y
+
+
+def kmass_lumped(mesh, k, d, stream=None):
+    check(lib().tn_kmass_lumped(mesh.h, dtype_code(k.dtype), _ptr(k), _ptr(d), _stream_ptr(stream)))
+    return d
+
+
+def prolongate_add(coarse, fine, xc, xf, ncomp=3, stream=None):
+    check(lib().tn_prolongate_add(coarse.h, fine.h, dtype_code(xc.dtype), ncomp, _ptr(xc), _ptr(xf),
+                                  _stream_ptr(stream)))
+    return xf
+
+
+def restrict(fine, coarse, xf, xc, ncomp=3, stream=None):
+    check(lib().tn_restrict(fine.h, coarse.h, dtype_code(xf.dtype), ncomp, _ptr(xf), _ptr(xc), _stream_ptr(stream)))
+    return xc
+
+
+def boundary_assign(mesh, dst, src, which, ncomp=3, stream=None):
+    check(lib().tn_boundary_assign(mesh.h, dtype_code(dst.dtype), ncomp, _ptr(dst), _ptr(src), which,
+                                   _stream_ptr(stream)))
+
+
+def freeslip_project(mesh, v, which, stream=None):
+    check(lib().tn_freeslip_project(mesh.h, dtype_code(v.dtype), _ptr(v), which, _stream_ptr(stream)))
+
+
+class Stokes:
+    """EpsDivDivStokes drop-in: [A B^T; B 0] on (velocity mesh, pressure mesh)."""
+
+    def __init__(self, vel, pre, A):
+        self.vel, self.pre, self.A = vel, pre, A
+        h = C.c_void_p()
+        check(lib().tn_stokes_create(vel.h, pre.h, A.h, C.byref(h)))
+        self.h = h
+
+    def apply(self, u, p, yu, yp, stream=None):
+        check(lib().tn_stokes_apply(self.h, _ptr(u), _ptr(p), _ptr(yu), _ptr(yp), _stream_ptr(stream)))
+
+    def div(self, u, p, stream=None):
+        check(lib().tn_div_apply(self.h, _ptr(u), _ptr(p), _stream_ptr(stream)))
+        return p
+
+    def grad(self, p, u, apply_mode=TN_REPLACE, comm_mode=TN_COMM_ADDITIVE, stream=None):
+        check(lib().tn_grad_apply(self.h, _ptr(p), _ptr(u), apply_mode, comm_mode, _stream_ptr(stream)))
+        return u
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().tn_stokes_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def gmg_options(cheby_order=2, cheby_iterations=3, power_iterations=10, num_cycles=1, coarse_max_iterations=1000,
+                coarse_rtol=1e-6, coarse_atol=1e-16, seed=12345):
+    return GmgOptions(cheby_order, cheby_iterations, power_iterations, num_cycles, coarse_max_iterations, coarse_rtol,
+                      coarse_atol, seed)
+
+
+class Gmg:
+    """Multigrid V-cycle over epsilon operators ordered coarse -> fine."""
+
+    def __init__(self, ops, options=None, stream=None):
+        self.ops = list(ops)
+        arr = (C.c_void_p * len(self.ops))(*[o.h.value for o in self.ops])
+        self.opts = options or gmg_options()
+        h = C.c_void_p()
+        check(lib().tn_gmg_create(len(self.ops), arr, C.byref(self.opts), _stream_ptr(stream), C.byref(h)))
+        self.h = h
+
+    def solve(self, x, b, stream=None):
+        check(lib().tn_gmg_solve(self.h, _ptr(x), _ptr(b), _stream_ptr(stream)))
+        return x
+
+    def lambda_max(self, level):
+        v = C.c_double()
+        check(lib().tn_gmg_get_lambda(self.h, level, C.byref(v)))
+        return v.value
+
+    def set_lambda_max(self, level, value):
+        check(lib().tn_gmg_set_lambda(self.h, level, value))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().tn_gmg_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def _info_dict(info, hist):
+    n = min(info.history_len, len(hist))
+    return {"iterations": info.iterations, "initial_residual": info.initial_residual,
+            "final_residual": info.final_residual, "seconds": info.seconds, "history": hist[:n].copy(),
+            "restarts": info.restarts, "prec_skipped": info.prec_skipped}
+
+
+def pcg_solve(A, x, b, prec=None, max_iterations=100, rtol=1e-8, atol=1e-30, stream=None):
+    o = KrylovOptions(max_iterations, rtol, atol, 30)
+    info = SolveInfo()
+    hist = np.zeros(max_iterations + 2)
+    check(lib().tn_pcg_solve(A.h, prec.h if prec is not None else None, _ptr(x), _ptr(b), C.byref(o), C.byref(info),
+                             hist.ctypes.data, len(hist), _stream_ptr(stream)))
+    return _info_dict(info, hist)
+
+
+def stokes_solve(K, method, prec_u, schur_diag, u, p, bu, bp, max_iterations=100, rtol=1e-8, atol=1e-30,
+                 restart=30, stream=None):
+    o = KrylovOptions(max_iterations, rtol, atol, restart)
+    info = SolveInfo()
+    hist = np.zeros(max_iterations + 2)
+    check(lib().tn_stokes_solve(K.h, method, prec_u.h, _ptr(schur_diag), _ptr(u), _ptr(p), _ptr(bu), _ptr(bp),
+                                C.byref(o), C.byref(info), hist.ctypes.data, len(hist), _stream_ptr(stream)))
+    return _info_dict(info, hist)

@@ -1,10 +1,14 @@
 // C ABI: mesh, level-1 operations and the epsilon-div-div operator.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
 
 #include "tn_internal.hpp"
+#include "tn_eps_march.hpp"
+#include "tn_mg.hpp"
+#include "tn_objects.hpp"
 
 namespace tn {
 namespace {
@@ -24,20 +28,6 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
     return 0;
 
 using tn::MeshLevel;
-
-// ------------------------------------------------------------------ epsilon operator object
-struct tn_eps {
-    const tn_mesh* mesh = nullptr;
-    int dtype = TN_F64;
-    const void* eta = nullptr;
-    int bc_cmb = TN_NEUMANN, bc_surf = TN_NEUMANN;
-    int apply_mode = TN_REPLACE, comm_mode = TN_COMM_ADDITIVE;
-    bool diagonal = false;
-    double penalty_eps = 1.0;
-    // fs/fs rotation penalty: y += eps * (G^{-1} beta) x p, beta = sum_owned p x u
-    tn::DevBuf<double> pen;  // [0..3) beta, [4..13) S = eps G^{-1}
-    bool penalty_ready = false;
-};
 
 namespace {
 
@@ -92,6 +82,17 @@ tn::Geom<float> geom_of<float>(const MeshLevel& m)
     return {m.d_coords_f.p, m.d_radii_f.p, m.d_geo_f.p, m.n, m.nr};
 }
 
+int march_seg(int n)
+{
+    static const int env = [] {
+        const char* e = std::getenv("TN_MARCH_SEG");
+        return e ? std::atoi(e) : 0;
+    }();
+    const int target = env > 0 ? env : 16;
+    const int nseg = (n + target - 1) / target;
+    return (n + nseg - 1) / nseg;
+}
+
 template <typename T>
 void eps_apply_t(tn_eps* op, const void* src_v, void* dst_v, hipStream_t s)
 {
@@ -99,26 +100,68 @@ void eps_apply_t(tn_eps* op, const void* src_v, void* dst_v, hipStream_t s)
     const T* src = static_cast<const T*>(src_v);
     T* dst = static_cast<T*>(dst_v);
     const T* eta = static_cast<const T*>(op->eta);
-    TN_REQUIRE(src && dst && eta, "eps apply: null pointer");
+    TN_REQUIRE(src && dst, "eps apply: null pointer");
     TN_REQUIRE((const void*)src != (void*)dst, "eps apply: src and dst must differ");
     const bool penalty = !op->diagonal && op->bc_cmb == TN_FREESLIP && op->bc_surf == TN_FREESLIP;
+    const bool add = op->apply_mode == TN_ADD;
     if (penalty) {
         if (!op->penalty_ready) build_penalty(op);
         tn::launch_fill<double>(op->pen.p, 4, 0.0, s);
         tn::launch_penalty_moment<T>(m, src, op->pen.p, s);
-        tn::launch_penalty_init<T>(m, op->pen.p, op->pen.p + 4, dst, op->apply_mode == TN_ADD, s);
-    } else if (op->apply_mode == TN_REPLACE) {
-        tn::launch_fill<T>(dst, 3 * m.N, T(0), s);
     }
-    tn::EpsLaunch cfg;
-    cfg.bc_cmb = op->bc_cmb;
-    cfg.bc_surf = op->bc_surf;
-    cfg.diagonal = op->diagonal;
-    tn::launch_eps_apply<T>(geom_of<T>(m), eta, src, dst, cfg, s);
+    if (op->stored_dtype < 0 && tn::march_supported(m.nr)) {
+        // matrix-free, atomic-free: the kernel writes every node copy once
+        TN_REQUIRE(eta, "eps apply: viscosity not set");
+        const bool need_blk = op->bc_cmb != TN_NEUMANN || op->bc_surf != TN_NEUMANN;
+        if (need_blk && !op->blk_ready) {
+            const size_t bytes = sizeof(T) * 2 * 10 * (size_t)m.n * m.n * 3;
+            if (op->blk.n != bytes) op->blk.alloc(bytes);
+            tn::launch_boundary_blocks<T>(geom_of<T>(m), eta, op->bc_cmb, op->bc_surf, (T*)op->blk.p, s);
+            op->blk_ready = true;
+        }
+        const tn::Geom<T> g = geom_of<T>(m);
+        tn::MarchArgs<T> a{src, dst, eta, g.geo, g.coords, g.radii, m.d_coords.p, m.d_radii.p,
+                           need_blk ? (const T*)op->blk.p : nullptr, penalty ? op->pen.p : nullptr,
+                           m.n, m.nr, march_seg(m.n), op->bc_cmb, op->bc_surf, add ? 1 : 0};
+        tn::launch_eps_march<T>(a, op->diagonal, s);
+    } else {
+        if (penalty)
+            tn::launch_penalty_init<T>(m, op->pen.p, op->pen.p + 4, dst, add, s);
+        else if (!add)
+            tn::launch_fill<T>(dst, 3 * m.N, T(0), s);
+        if (op->stored_dtype >= 0) {
+            if (op->stored_dtype == TN_F64) {
+                tn::StoredArgs<T, double> a{(const double*)op->stored.p, src, dst, m.d_coords.p, m.n, m.nr,
+                                            op->bc_cmb, op->bc_surf};
+                tn::launch_stored_apply<T, double>(a, op->diagonal, s);
+            } else {
+                tn::StoredArgs<T, float> a{(const float*)op->stored.p, src, dst, m.d_coords.p, m.n, m.nr,
+                                           op->bc_cmb, op->bc_surf};
+                tn::launch_stored_apply<T, float>(a, op->diagonal, s);
+            }
+        } else {
+            TN_REQUIRE(eta, "eps apply: viscosity not set");
+            tn::EpsLaunch cfg;
+            cfg.bc_cmb = op->bc_cmb;
+            cfg.bc_surf = op->bc_surf;
+            cfg.diagonal = op->diagonal;
+            tn::launch_eps_apply<T>(geom_of<T>(m), eta, src, dst, cfg, s);
+        }
+    }
     if (op->comm_mode == TN_COMM_ADDITIVE) tn::launch_interface_sum<T>(m, 3, dst, s);
 }
 
 }  // namespace
+
+namespace tn {
+void eps_apply(tn_eps* op, const void* src, void* dst, hipStream_t s)
+{
+    if (op->dtype == TN_F64)
+        eps_apply_t<double>(op, src, dst, s);
+    else
+        eps_apply_t<float>(op, src, dst, s);
+}
+}  // namespace tn
 
 extern "C" {
 
@@ -336,6 +379,7 @@ int tn_eps_set_boundary_conditions(tn_eps* op, int bc_cmb, int bc_surf)
     TN_REQUIRE(bc_cmb >= 0 && bc_cmb <= 2 && bc_surf >= 0 && bc_surf <= 2, "invalid boundary condition");
     op->bc_cmb = bc_cmb;
     op->bc_surf = bc_surf;
+    op->blk_ready = false;
     TN_API_END
 }
 
@@ -344,6 +388,7 @@ int tn_eps_set_eta(tn_eps* op, const void* eta)
     TN_API_BEGIN
     TN_REQUIRE(op, "null operator");
     op->eta = eta;
+    op->blk_ready = false;
     TN_API_END
 }
 
@@ -351,10 +396,84 @@ int tn_eps_apply(tn_eps* op, const void* src, void* dst, tn_stream_t s)
 {
     TN_API_BEGIN
     TN_REQUIRE(op, "null operator");
-    if (op->dtype == TN_F64)
-        eps_apply_t<double>(op, src, dst, (hipStream_t)s);
-    else
-        eps_apply_t<float>(op, src, dst, (hipStream_t)s);
+    tn::eps_apply(op, src, dst, (hipStream_t)s);
+    TN_API_END
+}
+
+int tn_eps_set_penalty_epsilon(tn_eps* op, double eps)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op, "null operator");
+    op->penalty_eps = eps;
+    op->penalty_ready = false;
+    TN_API_END
+}
+
+int tn_eps_gca_from(tn_eps* coarse, const tn_eps* fine, int store_dtype, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(coarse && fine, "null operator");
+    TN_REQUIRE(store_dtype == TN_F64 || store_dtype == TN_F32, "store dtype must be TN_F64 or TN_F32");
+    auto& mc = const_cast<tn_mesh*>(coarse->mesh)->lvl.dev();
+    auto& mf = const_cast<tn_mesh*>(fine->mesh)->lvl.dev();
+    TN_REQUIRE(2 * (mc.n - 1) == mf.n - 1 && 2 * (mc.nr - 1) == mf.nr - 1,
+               "GCA: fine mesh must be the uniform refinement of the coarse mesh");
+    const int64_t bytes = coarse->cells() * 2 * tn::PACKED * (store_dtype == TN_F64 ? 8 : 4);
+    coarse->stored.alloc((size_t)bytes);
+    coarse->stored_dtype = store_dtype;
+    const bool fine_geo = fine->stored_dtype < 0;
+    TN_REQUIRE(!fine_geo || fine->eta, "GCA: fine operator has no viscosity");
+    auto go = [&](auto* storec, auto* eta_f, auto* stored_f) {
+        using S = std::remove_pointer_t<decltype(storec)>;
+        using ET = std::remove_const_t<std::remove_pointer_t<decltype(eta_f)>>;
+        using SF = std::remove_const_t<std::remove_pointer_t<decltype(stored_f)>>;
+        tn::GcaArgs<ET, S, SF> a{mc.n, mc.nr, mf.n, mf.nr, mf.d_geo.p, mf.d_radii.p, eta_f, stored_f, storec};
+        tn::launch_gca_build<ET, S, SF>(a, fine_geo, (hipStream_t)s);
+    };
+    const bool f64e = fine->dtype == TN_F64;
+    const bool f64sf = fine->stored_dtype == TN_F64 || fine->stored_dtype < 0;
+    if (store_dtype == TN_F64) {
+        double* sc = (double*)coarse->stored.p;
+        if (f64e && f64sf) go(sc, (const double*)fine->eta, (const double*)fine->stored.p);
+        else if (f64e) go(sc, (const double*)fine->eta, (const float*)fine->stored.p);
+        else if (f64sf) go(sc, (const float*)fine->eta, (const double*)fine->stored.p);
+        else go(sc, (const float*)fine->eta, (const float*)fine->stored.p);
+    } else {
+        float* sc = (float*)coarse->stored.p;
+        if (f64e && f64sf) go(sc, (const double*)fine->eta, (const double*)fine->stored.p);
+        else if (f64e) go(sc, (const double*)fine->eta, (const float*)fine->stored.p);
+        else if (f64sf) go(sc, (const float*)fine->eta, (const double*)fine->stored.p);
+        else go(sc, (const float*)fine->eta, (const float*)fine->stored.p);
+    }
+    TN_API_END
+}
+
+int tn_eps_get_local_matrix(const tn_eps* op, int sd, int x, int y, int r, int wedge, double* out)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op && out, "null pointer");
+    TN_REQUIRE(op->stored_dtype >= 0, "operator has no stored matrices");
+    const auto& m = op->mesh->lvl;
+    TN_REQUIRE(sd >= 0 && sd < 10 && x >= 0 && x < m.n - 1 && y >= 0 && y < m.n - 1 && r >= 0 && r < m.nr - 1 &&
+                   (wedge == 0 || wedge == 1),
+               "local matrix index out of range");
+    const int64_t cell = (((int64_t)sd * (m.n - 1) + x) * (m.n - 1) + y) * (m.nr - 1) + r;
+    const int64_t nc = op->cells();
+    const size_t es = op->stored_dtype == TN_F64 ? 8 : 4;
+    std::vector<double> packed(tn::PACKED);
+    for (int k = 0; k < tn::PACKED; ++k) {
+        const size_t off = ((size_t)(wedge * tn::PACKED + k) * nc + cell) * es;
+        if (es == 8) {
+            TN_HIP(hipMemcpy(&packed[k], op->stored.p + off, 8, hipMemcpyDeviceToHost));
+        } else {
+            float f;
+            TN_HIP(hipMemcpy(&f, op->stored.p + off, 4, hipMemcpyDeviceToHost));
+            packed[k] = f;
+        }
+    }
+    int k = 0;
+    for (int i = 0; i < 18; ++i)
+        for (int j = i; j < 18; ++j, ++k) out[i * 18 + j] = out[j * 18 + i] = packed[k];
     TN_API_END
 }
 

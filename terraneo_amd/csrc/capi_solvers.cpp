@@ -1,0 +1,341 @@
+// C ABI: geometric multigrid for the viscous block and the Krylov drivers
+// (PCG for the viscous operator; MINRES / FGMRES for the Stokes system).
+#include <chrono>
+#include <memory>
+
+#include "tn_objects.hpp"
+#include "tn_solvers.hpp"
+
+namespace tn {
+void set_last_error(const std::string& msg);
+}  // namespace tn
+
+#define TN_API_BEGIN try {
+#define TN_API_END                    \
+    }                                 \
+    catch (const std::exception& e)   \
+    {                                 \
+        tn::set_last_error(e.what()); \
+        return -1;                    \
+    }                                 \
+    return 0;
+
+using namespace tn;
+
+namespace {
+
+Field field_of(const tn_mesh* m, int dtype, int ncomp, const void* p)
+{
+    Field f;
+    f.p = const_cast<void*>(p);
+    f.dtype = dtype;
+    f.ncomp = ncomp;
+    f.m = &const_cast<tn_mesh*>(m)->lvl.dev();
+    return f;
+}
+
+BVec vec1(const Field& f)
+{
+    BVec v;
+    v.b[0] = f;
+    v.nb = 1;
+    return v;
+}
+
+BVec vec2(const Field& u, const Field& p)
+{
+    BVec v;
+    v.b[0] = u;
+    v.b[1] = p;
+    v.nb = 2;
+    return v;
+}
+
+struct EpsOp : LinOp {
+    tn_eps* op;
+    explicit EpsOp(tn_eps* o) : op(o) {}
+    void apply(Ctx& c, const BVec& x, const BVec& y) override { eps_apply(op, x.b[0].p, y.b[0].p, c.s); }
+};
+
+struct StokesOp : LinOp {
+    tn_stokes* K;
+    explicit StokesOp(tn_stokes* k) : K(k) {}
+    void apply(Ctx& c, const BVec& x, const BVec& y) override
+    {
+        stokes_apply(K, x.b[0].p, x.b[1].p, y.b[0].p, y.b[1].p, c.s);
+    }
+};
+
+struct NullOp : LinOp {
+    void apply(Ctx& c, const BVec&, const BVec& y) override { v_fill(c, y, 0.0); }
+};
+
+}  // namespace
+
+// Geometric multigrid on the epsilon operator (viscous block): one tn_eps
+// per level (coarse .. fine), inverse diagonals, Chebyshev smoothers,
+// constant transfers, PCG on the coarsest level (reference app/test
+// construction: mantlecirculation.cpp:556-645, test_epsilon_divdiv_stokes_assess.cpp:554-660).
+struct tn_gmg {
+    int dtype = TN_F64;
+    std::vector<tn_eps*> ops;
+    std::vector<std::unique_ptr<EpsOp>> lops;
+    VecStore store;
+    std::vector<BVec> invd, ta, tb, tmp, r, e, ctmp;
+    std::vector<std::unique_ptr<Chebyshev>> sm;
+    std::unique_ptr<PCG> coarse;
+    std::unique_ptr<Multigrid> mg;
+    Ctx ctx;
+};
+
+namespace {
+
+void gmg_setup(tn_gmg* g, const tn_gmg_options& o, hipStream_t s)
+{
+    g->ctx.s = s;
+    Ctx& c = g->ctx;
+    const int L = (int)g->ops.size();
+    for (int l = 0; l < L; ++l) {
+        tn_eps* op = g->ops[l];
+        g->lops.emplace_back(new EpsOp(op));
+        const BVec like = vec1(field_of(op->mesh, op->dtype, 3, nullptr));
+        BVec ones = g->store.make(like);
+        g->invd.push_back(g->store.make(like));
+        v_fill(c, ones, 1.0);
+        const bool dg = op->diagonal;
+        const int am = op->apply_mode, cm = op->comm_mode;
+        op->diagonal = true;
+        op->apply_mode = TN_REPLACE;
+        op->comm_mode = TN_COMM_ADDITIVE;
+        eps_apply(op, ones.b[0].p, g->invd.back().b[0].p, s);
+        op->diagonal = dg;
+        op->apply_mode = am;
+        op->comm_mode = cm;
+        v_invert(c, g->invd.back());
+        g->ta.push_back(g->store.make(like));
+        g->tb.push_back(g->store.make(like));
+        g->tmp.push_back(g->store.make(like));
+        if (l < L - 1) {
+            g->r.push_back(g->store.make(like));
+            g->e.push_back(g->store.make(like));
+        }
+        g->sm.emplace_back(new Chebyshev(o.cheby_order, o.cheby_iterations, g->invd.back(), g->ta.back(), g->tb.back(),
+                                         o.power_iterations, o.seed + 1000003ull * l));
+    }
+    const BVec like0 = vec1(field_of(g->ops[0]->mesh, g->ops[0]->dtype, 3, nullptr));
+    for (int i = 0; i < 4; ++i) g->ctmp.push_back(g->store.make(like0));
+    KrylovParams kp;
+    kp.max_iterations = o.coarse_max_iterations;
+    kp.rtol = o.coarse_rtol;
+    kp.atol = o.coarse_atol;
+    g->coarse.reset(new PCG(kp, g->ctmp, nullptr));
+    std::vector<LinOp*> lo;
+    std::vector<Transfer> tr;
+    std::vector<Solver*> sms;
+    for (int l = 0; l < L; ++l) {
+        lo.push_back(g->lops[l].get());
+        sms.push_back(g->sm[l].get());
+        if (l < L - 1)
+            tr.push_back(Transfer{&const_cast<tn_mesh*>(g->ops[l]->mesh)->lvl, &const_cast<tn_mesh*>(g->ops[l + 1]->mesh)->lvl});
+    }
+    g->mg.reset(new Multigrid(lo, tr, sms, g->coarse.get(), g->tmp, g->r, g->e, o.num_cycles));
+}
+
+void fill_info(tn_solve_info* info, const IterInfo& it, double secs, double* hist, int cap)
+{
+    if (!info) return;
+    info->iterations = it.iterations;
+    info->initial_residual = it.initial_residual;
+    info->final_residual = it.final_residual;
+    info->seconds = secs;
+    info->history_len = (int)it.history.size();
+    if (hist)
+        for (int i = 0; i < cap && i < (int)it.history.size(); ++i) hist[i] = it.history[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+int tn_gmg_create(int nlevels, tn_eps* const* ops, const tn_gmg_options* opts, tn_stream_t s, tn_gmg** out)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(ops && opts && out && nlevels >= 2, "need at least 2 levels");
+    auto g = std::make_unique<tn_gmg>();
+    g->dtype = ops[0]->dtype;
+    for (int l = 0; l < nlevels; ++l) {
+        TN_REQUIRE(ops[l] && ops[l]->dtype == g->dtype, "all levels must share one dtype");
+        if (l > 0) {
+            const auto& mc = ops[l - 1]->mesh->lvl;
+            const auto& mf = ops[l]->mesh->lvl;
+            TN_REQUIRE(2 * (mc.n - 1) == mf.n - 1 && 2 * (mc.nr - 1) == mf.nr - 1,
+                       "levels must be uniform refinements (coarse first)");
+        }
+        g->ops.push_back(ops[l]);
+    }
+    gmg_setup(g.get(), *opts, (hipStream_t)s);
+    *out = g.release();
+    TN_API_END
+}
+
+int tn_gmg_destroy(tn_gmg* g)
+{
+    TN_API_BEGIN
+    delete g;
+    TN_API_END
+}
+
+// x <- V-cycle(s) applied to A x = b starting from the content of x
+int tn_gmg_solve(tn_gmg* g, void* x, const void* b, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(g && x && b, "null pointer");
+    g->ctx.s = (hipStream_t)s;
+    tn_eps* A = g->ops.back();
+    const BVec X = vec1(field_of(A->mesh, A->dtype, 3, x)), B = vec1(field_of(A->mesh, A->dtype, 3, b));
+    g->mg->solve(g->ctx, *g->lops.back(), X, B);
+    TN_API_END
+}
+
+int tn_gmg_get_lambda(const tn_gmg* g, int level, double* lambda)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(g && lambda && level >= 0 && level < (int)g->sm.size(), "bad level");
+    *lambda = g->sm[level]->lambda_max();
+    TN_API_END
+}
+
+int tn_gmg_set_lambda(tn_gmg* g, int level, double lambda)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(g && level >= 0 && level < (int)g->sm.size(), "bad level");
+    g->sm[level]->set_lambda_max(lambda);
+    TN_API_END
+}
+
+int tn_pcg_solve(tn_eps* A, tn_gmg* prec, void* x, const void* b, const tn_krylov_options* o, tn_solve_info* info,
+                 double* history, int history_cap, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(A && x && b && o, "null pointer");
+    Ctx c;
+    c.s = (hipStream_t)s;
+    VecStore st;
+    const BVec X = vec1(field_of(A->mesh, A->dtype, 3, x)), B = vec1(field_of(A->mesh, A->dtype, 3, b));
+    std::vector<BVec> t;
+    for (int i = 0; i < 4; ++i) t.push_back(st.make(X));
+    KrylovParams kp{o->max_iterations, o->rtol, o->atol, o->restart};
+    EpsOp op(A);
+    struct GmgPrec : Solver {
+        tn_gmg* g;
+        void solve(Ctx& c, LinOp&, const BVec& x, const BVec& b) override
+        {
+            g->ctx.s = c.s;
+            g->mg->solve(g->ctx, *g->lops.back(), x, b);
+        }
+    } gp;
+    gp.g = prec;
+    PCG pcg(kp, t, prec ? &gp : nullptr);
+    TN_HIP(hipStreamSynchronize(c.s));
+    const auto t0 = std::chrono::steady_clock::now();
+    pcg.solve(c, op, X, B);
+    TN_HIP(hipStreamSynchronize(c.s));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fill_info(info, pcg.info, secs, history, history_cap);
+    TN_API_END
+}
+
+// Stokes solve with a velocity-block multigrid (any dtype; a precision bridge
+// is inserted when it differs from the Stokes operator) and a diagonal Schur
+// complement preconditioner (inverse of schur_diag, e.g. lumped M_p(1/eta)).
+int tn_stokes_solve(tn_stokes* K, int method, tn_gmg* prec_u, const void* schur_diag, void* u, void* p,
+                    const void* bu, const void* bp, const tn_krylov_options* o, tn_solve_info* info, double* history,
+                    int history_cap, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(K && prec_u && schur_diag && u && p && bu && bp && o, "null pointer");
+    Ctx c;
+    c.s = (hipStream_t)s;
+    VecStore st;
+    const int dt = K->A->dtype;
+    const Field fu = field_of(K->vel, dt, 3, u), fp = field_of(K->pre, dt, 1, p);
+    const BVec X = vec2(fu, fp);
+    const BVec B = vec2(field_of(K->vel, dt, 3, bu), field_of(K->pre, dt, 1, bp));
+    // Schur: DiagonalSolver inverts its diagonal (diagonal_solver.hpp:34-40)
+    BVec sinv = st.make(vec1(fp));
+    {
+        BVec sd = vec1(field_of(K->pre, dt, 1, schur_diag));
+        v_assign(c, sinv, sd);
+        v_invert(c, sinv);
+    }
+    DiagonalSolver schur(sinv);
+    StokesOp Kop(K);
+    NullOp null;
+    // velocity block preconditioner, possibly in another precision
+    struct GmgPrec : Solver {
+        tn_gmg* g;
+        void solve(Ctx& c, LinOp&, const BVec& x, const BVec& b) override
+        {
+            g->ctx.s = c.s;
+            g->mg->solve(g->ctx, *g->lops.back(), x, b);
+        }
+    } gp;
+    gp.g = prec_u;
+    std::unique_ptr<PrecisionBridge> bridge;
+    Solver* pu = &gp;
+    if (prec_u->dtype != dt) {
+        const BVec ui = st.make_like(vec1(fu), prec_u->dtype), bi = st.make_like(vec1(fu), prec_u->dtype);
+        bridge.reset(new PrecisionBridge(&gp, prec_u->lops.back().get(), ui, bi));
+        pu = bridge.get();
+    }
+    EpsOp A11(K->A);
+    KrylovParams kp{o->max_iterations, o->rtol, o->atol, o->restart};
+    IterInfo it;
+    int restarts = 0, skipped = 0;
+    TN_HIP(hipStreamSynchronize(c.s));
+    const auto t0 = std::chrono::steady_clock::now();
+    if (method == TN_MINRES_BLOCKDIAG) {
+        BlockDiagonal P(&A11, pu, &null, &schur);
+        std::vector<BVec> t;
+        for (int i = 0; i < 7; ++i) t.push_back(st.make(X));
+        PMINRES solver(kp, t, &P);
+        solver.solve(c, Kop, X, B);
+        it = solver.info;
+    } else {
+        TN_REQUIRE(method == TN_FGMRES_BLOCKTRI || method == TN_FGMRES_BLOCKDIAG, "unknown Stokes solver method");
+        std::vector<BVec> t;
+        for (int i = 0; i < 2 * kp.restart + 4; ++i) t.push_back(st.make(X));
+        std::unique_ptr<Solver> P;
+        BVec ttmp = st.make(X);
+        struct BtOp : LinOp {  // block_12: B^T p, replace + additive communication
+            tn_stokes* K;
+            void apply(Ctx& c, const BVec& x, const BVec& y) override
+            {
+                if (K->A->dtype == TN_F64)
+                    grad_t<double>(K, x.b[0].p, y.b[0].p, false, true, c.s);
+                else
+                    grad_t<float>(K, x.b[0].p, y.b[0].p, false, true, c.s);
+            }
+        } bt;
+        bt.K = K;
+        if (method == TN_FGMRES_BLOCKTRI)
+            P.reset(new BlockTriangular(&A11, pu, &null, &schur, &bt, ttmp));
+        else
+            P.reset(new BlockDiagonal(&A11, pu, &null, &schur));
+        FGMRES solver(kp, t, P.get(), true);
+        solver.solve(c, Kop, X, B);
+        it = solver.info;
+        restarts = solver.restarts_breakdown + solver.restarts_stagnation;
+        skipped = solver.prec_skipped;
+    }
+    TN_HIP(hipStreamSynchronize(c.s));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fill_info(info, it, secs, history, history_cap);
+    if (info) {
+        info->restarts = restarts;
+        info->prec_skipped = skipped;
+    }
+    TN_API_END
+}
+
+}  // extern "C"

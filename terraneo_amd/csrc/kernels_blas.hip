@@ -146,6 +146,80 @@ __global__ void k_penalty_init(const double* __restrict__ beta, const double* __
     }
 }
 
+template <typename T>
+__global__ void k_mul(T* __restrict__ y, const T* __restrict__ d, int64_t count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] *= d[i];
+}
+
+template <typename T>
+__global__ void k_invert(T* __restrict__ y, int64_t count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = T(1) / y[i];
+}
+
+template <typename TD, typename TS>
+__global__ void k_cast(TD* __restrict__ y, const TS* __restrict__ x, int64_t count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (TD)x[i];
+}
+
+// counter-based uniform [0,1) values (SplitMix64 of seed and global index):
+// the same value for every run, independent of the launch shape
+__device__ inline double splitmix_unit(uint64_t seed, uint64_t i)
+{
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+template <typename T>
+__global__ void k_randomize(T* __restrict__ y, int64_t count, uint64_t seed)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (T)splitmix_unit(seed, (uint64_t)i);
+}
+
+template <typename T>
+__global__ void k_nonfinite(const T* __restrict__ y, int64_t count, int* flag)
+{
+    int bad = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        bad |= !isfinite((double)y[i]);
+    if (bad) atomicOr(flag, 1);
+}
+
+// out[k] += sum_owned a . b_k for k < nb (multi-dot, one pass over a)
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void k_multidot(const T* __restrict__ a, MultiDotArgs<T> bs,
+                                                  const uint8_t* __restrict__ owned_lat, int64_t N, int nr, int ncomp,
+                                                  double* out)
+{
+    double s[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) s[k] = 0.0;
+    const int64_t total = N * ncomp;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (!owned_lat[(i % N) / nr]) continue;
+        const double av = (double)a[i];
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+            if (k < bs.nb) s[k] += av * (double)bs.b[k][i];
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        if (k >= bs.nb) break;
+        block_accumulate(s[k], out + k);
+        __syncthreads();
+    }
+}
+
 inline unsigned grid_for(int64_t count, int block = 256, int64_t cap = 16384)
 {
     int64_t g = (count + block - 1) / block;
@@ -221,13 +295,72 @@ void launch_penalty_init(const MeshLevel& m, const double* beta_dev, const doubl
     TN_HIP(hipGetLastError());
 }
 
+template <typename T>
+void launch_mul(T* y, const T* d, int64_t count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mul<T>, dim3(grid_for(count)), dim3(256), 0, s, y, d, count);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_invert(T* y, int64_t count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_invert<T>, dim3(grid_for(count)), dim3(256), 0, s, y, count);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename TD, typename TS>
+void launch_cast(TD* y, const TS* x, int64_t count, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cast<TD, TS>), dim3(grid_for(count)), dim3(256), 0, s, y, x, count);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_randomize(T* y, int64_t count, uint64_t seed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_randomize<T>, dim3(grid_for(count)), dim3(256), 0, s, y, count, seed);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_nonfinite(const T* y, int64_t count, int* flag_dev, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nonfinite<T>, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, y, count, flag_dev);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_multidot(const MeshLevel& m, int ncomp, const T* a, const MultiDotArgs<T>& bs, double* out_dev,
+                     hipStream_t s)
+{
+    TN_REQUIRE(bs.nb >= 1 && bs.nb <= MultiDotArgs<T>::MAX, "multidot: 1..16 vectors");
+    const dim3 gr(grid_for(m.N * ncomp, 256, 1024)), bl(256);
+    if (bs.nb <= 4)
+        hipLaunchKernelGGL((k_multidot<T, 4>), gr, bl, 0, s, a, bs, m.d_owned_lat.p, m.N, m.nr, ncomp, out_dev);
+    else if (bs.nb <= 8)
+        hipLaunchKernelGGL((k_multidot<T, 8>), gr, bl, 0, s, a, bs, m.d_owned_lat.p, m.N, m.nr, ncomp, out_dev);
+    else
+        hipLaunchKernelGGL((k_multidot<T, 16>), gr, bl, 0, s, a, bs, m.d_owned_lat.p, m.N, m.nr, ncomp, out_dev);
+    TN_HIP(hipGetLastError());
+}
+
+template void launch_fill<int>(int*, int64_t, int, hipStream_t);
+template void launch_cast<double, float>(double*, const float*, int64_t, hipStream_t);
+template void launch_cast<float, double>(float*, const double*, int64_t, hipStream_t);
+
 #define TN_INST(T)                                                                                               \
     template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                                   \
     template void launch_lincomb<T>(int64_t, T*, int, const double*, const T* const*, double, hipStream_t);     \
     template void launch_dot_owned<T>(const MeshLevel&, int, const T*, const T*, double*, hipStream_t);          \
     template void launch_interface_sum<T>(const MeshLevel&, int, T*, hipStream_t);                               \
     template void launch_penalty_moment<T>(const MeshLevel&, const T*, double*, hipStream_t);                    \
-    template void launch_penalty_init<T>(const MeshLevel&, const double*, const double*, T*, bool, hipStream_t);
+    template void launch_penalty_init<T>(const MeshLevel&, const double*, const double*, T*, bool, hipStream_t); \
+    template void launch_mul<T>(T*, const T*, int64_t, hipStream_t);                                             \
+    template void launch_invert<T>(T*, int64_t, hipStream_t);                                                    \
+    template void launch_randomize<T>(T*, int64_t, uint64_t, hipStream_t);                                       \
+    template void launch_nonfinite<T>(const T*, int64_t, int*, hipStream_t);                                     \
+    template void launch_multidot<T>(const MeshLevel&, int, const T*, const MultiDotArgs<T>&, double*, hipStream_t);
 TN_INST(double)
 TN_INST(float)
 #undef TN_INST

@@ -1,0 +1,497 @@
+// Multigrid kernels: constant (linear-on-wedges) prolongation/restriction,
+// Galerkin coarse-grid (GCA) assembly and the stored-matrix operator apply.
+//
+// Reference counterparts:
+//   ProlongationVecConstant / RestrictionVecConstant   prolongation_constant.hpp:103-193,
+//                                                      restriction_constant.hpp:134-250,
+//                                                      weights grid_transfer_constant.hpp:25-89
+//   TwoGridGCA (constant interpolation)                linalg/solvers/gca/gca.hpp:154-531
+//   slow (stored local matrix) operator path           epsilon_divdiv_kerngen.hpp:823-1051
+//
+// Storage of coarse operators: the two 18x18 wedge matrices of every hex cell
+// are symmetric (P^T A P of a symmetric A), so only the upper triangle is
+// kept: 171 entries per wedge, dimension-wise dof order (d*6 + node), packed
+// row-major.  Entries are stored entry-major, [2*171][cells] with cells in
+// (diamond, x, y, r) order, r fastest: neighbouring lanes (radially adjacent
+// cells) read neighbouring addresses, so the matrix stream is coalesced.
+// The boundary masks and free-slip rotations are applied at apply time, as
+// the reference does for its stored matrices.
+#include "tn_internal.hpp"
+#include "tn_mg.hpp"
+
+namespace tn {
+namespace {
+
+__device__ constexpr int WOFF[2][6][3] = {{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}},
+                                         {{1, 1, 0}, {0, 1, 0}, {1, 0, 0}, {1, 1, 1}, {0, 1, 1}, {1, 0, 1}}};
+
+__host__ __device__ constexpr int pk(int i, int j)  // packed upper index, i <= j
+{
+    return i * 18 - (i * (i - 1)) / 2 + (j - i);
+}
+
+// ---------------------------------------------------------------- transfers
+
+// Prolongation (Add): every fine node gathers from its coarse parents.
+template <typename T>
+__global__ void k_prolongate_add(const T* __restrict__ xc, T* __restrict__ xf, int nf, int nrf, int nc, int nrc,
+                                 int64_t Nf, int64_t Nc, int ncomp)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= Nf) return;
+    const int rf = (int)(tid % nrf);
+    int64_t lat = tid / nrf;
+    const int yf = (int)(lat % nf);
+    lat /= nf;
+    const int xf_ = (int)(lat % nf);
+    const int d = (int)(lat / nf);
+    // lateral parents: vertex, edge midpoint (x, y or anti-diagonal)
+    int px[2], py[2], np;
+    T lw;
+    if (!(xf_ & 1) && !(yf & 1)) {
+        px[0] = xf_ >> 1; py[0] = yf >> 1; np = 1; lw = T(1);
+    } else if ((xf_ & 1) && !(yf & 1)) {
+        px[0] = xf_ >> 1; py[0] = yf >> 1; px[1] = px[0] + 1; py[1] = py[0]; np = 2; lw = T(0.5);
+    } else if (!(xf_ & 1) && (yf & 1)) {
+        px[0] = xf_ >> 1; py[0] = yf >> 1; px[1] = px[0]; py[1] = py[0] + 1; np = 2; lw = T(0.5);
+    } else {
+        px[0] = (xf_ >> 1) + 1; py[0] = yf >> 1; px[1] = xf_ >> 1; py[1] = (yf >> 1) + 1; np = 2; lw = T(0.5);
+    }
+    int pr[2], nq;
+    T rw;
+    if (!(rf & 1)) {
+        pr[0] = rf >> 1; nq = 1; rw = T(1);
+    } else {
+        pr[0] = rf >> 1; pr[1] = pr[0] + 1; nq = 2; rw = T(0.5);
+    }
+    const T w = lw * rw;
+    for (int c = 0; c < ncomp; ++c) {
+        T s = 0;
+        for (int a = 0; a < np; ++a)
+            for (int b = 0; b < nq; ++b)
+                s += xc[c * Nc + ((((int64_t)d * nc + px[a]) * nc + py[a]) * nrc + pr[b])];
+        xf[c * Nf + tid] += w * s;
+    }
+}
+
+__device__ inline double pc_lat_weight(int dx, int dy)
+{
+    if (dx == 0 && dy == 0) return 1.0;
+    if ((dy == 0 && (dx == 1 || dx == -1)) || (dx == 0 && (dy == 1 || dy == -1)) || (dx == -1 && dy == 1) ||
+        (dx == 1 && dy == -1))
+        return 0.5;
+    return 0.0;
+}
+
+// Restriction (Replace): coarse node = owned-masked weighted sum over the
+// 21-point stencil of fine nodes; interface sum follows on the coarse mesh.
+template <typename T>
+__global__ void k_restrict(const T* __restrict__ xf, T* __restrict__ xc, const uint8_t* __restrict__ owned_lat_f,
+                           int nf, int nrf, int nc, int nrc, int64_t Nf, int64_t Nc, int ncomp)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= Nc) return;
+    const int r = (int)(tid % nrc);
+    int64_t lat = tid / nrc;
+    const int y = (int)(lat % nc);
+    lat /= nc;
+    const int x = (int)(lat % nc);
+    const int d = (int)(lat / nc);
+    constexpr int ST[7][2] = {{-1, 0}, {-1, 1}, {0, 1}, {1, 0}, {1, -1}, {0, -1}, {0, 0}};
+    double acc[3] = {0, 0, 0};
+    for (int s = 0; s < 7; ++s) {
+        const int fx = 2 * x + ST[s][0], fy = 2 * y + ST[s][1];
+        if (fx < 0 || fx >= nf || fy < 0 || fy >= nf) continue;
+        const int64_t flat = ((int64_t)d * nf + fx) * nf + fy;
+        if (!owned_lat_f[flat]) continue;
+        const double lw = pc_lat_weight(ST[s][0], ST[s][1]);
+        for (int dr = -1; dr <= 1; ++dr) {
+            const int fr = 2 * r + dr;
+            if (fr < 0 || fr >= nrf) continue;
+            const double w = lw * (dr == 0 ? 1.0 : 0.5);
+            for (int c = 0; c < ncomp; ++c) acc[c] += w * (double)xf[c * Nf + flat * nrf + fr];
+        }
+    }
+    for (int c = 0; c < ncomp; ++c) xc[c * Nc + tid] = (T)acc[c];
+}
+
+// ---------------------------------------------------------------- GCA assembly
+
+// Interpolation from the 6 coarse wedge dofs onto the 6 dofs of one fine
+// wedge (gca.hpp:195-345, constant mode).  (fx, fy, fr) = fine hex cell,
+// fine wedge w, nrf = fine radial node count.  Reproduces the coded radial
+// bracket r_bot = r/2 (gca.hpp:213-216).
+__device__ inline void gca_local_P(int fx, int fy, int fr, int w, int nrf, double P[6][6])
+{
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) P[i][j] = 0.0;
+    for (int fl = 0; fl < 6; ++fl) {
+        const int ix = fx + WOFF[w][fl][0], iy = fy + WOFF[w][fl][1], ir = fr + WOFF[w][fl][2];
+        if (!(ix & 1) && !(iy & 1) && !(ir & 1)) {
+            P[fl][fl] = 1.0;
+            continue;
+        }
+        const int rb = ir < nrf - 1 ? ir / 2 : ir / 2 - 1, rt = rb + 1;
+        if (!(ix & 1) && !(iy & 1)) {
+            const int a = fl % 3;
+            P[fl][a] = 0.5;
+            P[fl][a + 3] = 0.5;
+            continue;
+        }
+        int x0, x1, y0, y1, cl[4];
+        if (!(ix & 1)) {
+            x0 = x1 = ix / 2; y0 = iy / 2; y1 = iy / 2 + 1;
+            cl[0] = 0; cl[1] = 2; cl[2] = 3; cl[3] = 5;
+        } else if (!(iy & 1)) {
+            x0 = ix / 2; x1 = ix / 2 + 1; y0 = y1 = iy / 2;
+            cl[0] = 0; cl[1] = 1; cl[2] = 3; cl[3] = 4;
+        } else {
+            x0 = ix / 2 + 1; x1 = ix / 2; y0 = iy / 2; y1 = iy / 2 + 1;
+            cl[0] = 1; cl[1] = 2; cl[2] = 4; cl[3] = 5;
+        }
+        auto wgt = [&](int xc, int yc, int rc) {
+            const int dr = ir - 2 * rc;
+            const double lw = pc_lat_weight(ix - 2 * xc, iy - 2 * yc);
+            return dr == 0 ? lw : ((dr == 1 || dr == -1) ? 0.5 * lw : 0.0);
+        };
+        P[fl][cl[0]] = wgt(x0, y0, rb);
+        P[fl][cl[1]] = wgt(x1, y1, rb);
+        P[fl][cl[2]] = wgt(x0, y0, rt);
+        P[fl][cl[3]] = wgt(x1, y1, rt);
+    }
+}
+
+// Fine-level 6x6 block (row component q, column component p) of the raw
+// epsilon wedge matrix assembled from geometry (one-point quadrature):
+//   A_(i,q),(j,p) = kwJ [ delta_pq g_i.g_j + g_i[p] g_j[q] - 2/3 g_i[q] g_j[p] ]
+template <typename T>
+__device__ inline void eps_block_geo(const T* __restrict__ geo_w, T r0, T r1, double ksum, int q, int p,
+                                     double A[6][6])
+{
+    const double half_dr = 0.5 * ((double)r1 - (double)r0), r_mid = 0.5 * ((double)r0 + (double)r1);
+    const double alpha = 1.0 / r_mid, beta = 1.0 / half_dr;
+    const double kwJ = ksum / 6.0 * r_mid * r_mid * half_dr * (double)geo_w[9];
+    double g[6][3];
+    for (int a = 0; a < 3; ++a) {
+        const double a1 = alpha * (double)geo_w[a], a2 = alpha * (double)geo_w[3 + a], z = beta * (double)geo_w[6 + a];
+        const double a0 = -(a1 + a2);
+        g[0][a] = a0 - z; g[1][a] = a1 - z; g[2][a] = a2 - z;
+        g[3][a] = a0 + z; g[4][a] = a1 + z; g[5][a] = a2 + z;
+    }
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            const double gg = g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2];
+            A[i][j] = kwJ * ((p == q ? gg : 0.0) + g[i][p] * g[j][q] - (2.0 / 3.0) * g[i][q] * g[j][p]);
+        }
+}
+
+template <typename S>
+__device__ inline void stored_block(const S* __restrict__ st, int64_t ncells, int64_t cell, int w, int q, int p,
+                                    double A[6][6])
+{
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            int r = i + 6 * q, c = j + 6 * p;
+            if (r > c) {
+                const int t = r;
+                r = c;
+                c = t;
+            }
+            A[i][j] = (double)st[(int64_t)(w * 171 + pk(r, c)) * ncells + cell];
+        }
+}
+
+// One thread per (coarse cell, coarse wedge, component block q <= p).
+// FINE_GEO: fine operator is matrix-free (raw local matrices from geometry);
+// otherwise it is read from the fine level's packed storage.
+template <typename T, typename S, typename SF, bool FINE_GEO>
+__global__ void k_gca_build(GcaArgs<T, S, SF> a)
+{
+    constexpr int QP[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ncc = 10LL * (a.nc - 1) * (a.nc - 1) * (a.nrc - 1);
+    if (tid >= ncc * 12) return;
+    const int blk = (int)(tid % 6);
+    const int W = (int)((tid / 6) % 2);
+    const int64_t cell = tid / 12;
+    const int q = QP[blk][0], p = QP[blk][1];
+    const int R = (int)(cell % (a.nrc - 1));
+    int64_t lat = cell / (a.nrc - 1);
+    const int Y = (int)(lat % (a.nc - 1));
+    lat /= (a.nc - 1);
+    const int X = (int)(lat % (a.nc - 1));
+    const int d = (int)(lat / (a.nc - 1));
+    constexpr int SH[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    // fine wedges of coarse wedge W (gca.hpp:353-370)
+    constexpr int LIST[2][8][2] = {{{0, 0}, {1, 0}, {2, 0}, {4, 0}, {5, 0}, {6, 0}, {0, 1}, {4, 1}},
+                                   {{1, 1}, {2, 1}, {3, 1}, {5, 1}, {6, 1}, {7, 1}, {3, 0}, {7, 0}}};
+    double out[6][6];
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) out[i][j] = 0.0;
+    const int ncf = a.nf - 1, ncrf = a.nrf - 1;
+    for (int k = 0; k < 8; ++k) {
+        const int h = LIST[W][k][0], w = LIST[W][k][1];
+        const int fx = 2 * X + SH[h][0], fy = 2 * Y + SH[h][1], fr = 2 * R + SH[h][2];
+        double P[6][6], A[6][6];
+        gca_local_P(fx, fy, fr, w, a.nrf, P);
+        const int64_t fcell = (((int64_t)d * ncf + fx) * ncf + fy) * ncrf + fr;
+        if constexpr (FINE_GEO) {
+            const double* geo_w = a.geo_f + ((((int64_t)d * ncf + fx) * ncf + fy) * 2 + w) * GEO_STRIDE;
+            double ksum = 0.0;
+            for (int j = 0; j < 6; ++j)
+                ksum += (double)a.eta_f[(((int64_t)d * a.nf + fx + WOFF[w][j][0]) * a.nf + fy + WOFF[w][j][1]) * a.nrf +
+                                        fr + WOFF[w][j][2]];
+            eps_block_geo<double>(geo_w, a.radii_f[fr], a.radii_f[fr + 1], ksum, q, p, A);
+        } else {
+            const int64_t ncellf = 10LL * ncf * ncf * ncrf;
+            stored_block<SF>(a.stored_f, ncellf, fcell, w, q, p, A);
+        }
+        // out += P^T A P
+        double T1[6][6];
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) {
+                double s = 0.0;
+                for (int l = 0; l < 6; ++l) s += A[i][l] * P[l][j];
+                T1[i][j] = s;
+            }
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) {
+                double s = 0.0;
+                for (int l = 0; l < 6; ++l) s += P[l][i] * T1[l][j];
+                out[i][j] += s;
+            }
+    }
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            const int r = i + 6 * q, c = j + 6 * p;
+            if (r <= c) a.stored_c[(int64_t)(W * 171 + pk(r, c)) * ncc + cell] = (S)out[i][j];
+        }
+}
+
+// ---------------------------------------------------------------- stored-matrix apply
+
+template <typename T, typename S, bool DIAG>
+__global__ __launch_bounds__(256) void k_stored_apply(StoredArgs<T, S> a)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ncell = 10LL * (a.n - 1) * (a.n - 1) * (a.nr - 1);
+    const bool active = tid < ncell;
+    const int64_t cell = active ? tid : ncell - 1;
+    const int ncr = a.nr - 1, ncl = a.n - 1;
+    const int rc = (int)(cell % ncr);
+    int64_t lat = cell / ncr;
+    const int y = (int)(lat % ncl);
+    lat /= ncl;
+    const int x = (int)(lat % ncl);
+    const int d = (int)(lat / ncl);
+    const int64_t N = 10LL * a.n * a.n * a.nr;
+    const int64_t lat00 = ((int64_t)d * a.n + x) * a.n + y;
+    // corner lateral ids: 00, 10, 01, 11
+    const int64_t latc[4] = {lat00, lat00 + a.n, lat00 + 1, lat00 + a.n + 1};
+    constexpr int WCORN[2][3] = {{0, 1, 2}, {3, 2, 1}};
+    const int bmode = rc == 0 ? a.bc_cmb : TN_NEUMANN;
+    const int tmode = rc == ncr - 1 ? a.bc_surf : TN_NEUMANN;
+    T yb[4][3], yt[4][3];
+    for (int c = 0; c < 4; ++c)
+        for (int k = 0; k < 3; ++k) yb[c][k] = yt[c][k] = T(0);
+    double nrm[4][3];
+    if (bmode == TN_FREESLIP || tmode == TN_FREESLIP) {
+        for (int c = 0; c < 4; ++c) {
+            const double* cc = a.coords + 3 * latc[c];
+            const double inv = 1.0 / sqrt(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
+            for (int k = 0; k < 3; ++k) nrm[c][k] = cc[k] * inv;
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        T xs[18];  // dimension-wise: comp*6 + node
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int c = WCORN[w][j % 3];
+            const int64_t ib = latc[c] * a.nr + rc + (j >= 3);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xs[k * 6 + j] = a.src[k * N + ib];
+        }
+        const S* m = a.stored + (int64_t)(w * 171) * ncell + cell;
+        auto A = [&](int i, int j) -> T { return (T)m[(int64_t)pk(i, j) * ncell]; };
+        T ys[18];
+        if constexpr (DIAG) {
+#pragma unroll
+            for (int i = 0; i < 18; ++i) ys[i] = A(i, i) * xs[i];
+            // free-slip boundary nodes: R^T diag(R A_bb R^T) R x
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                const int mode = j < 3 ? bmode : tmode;
+                if (mode != TN_FREESLIP) continue;
+                const int c = WCORN[w][j % 3];
+                T Ab[3][3];
+                for (int p = 0; p < 3; ++p)
+                    for (int q2 = 0; q2 < 3; ++q2) {
+                        const int r = p * 6 + j, cc = q2 * 6 + j;
+                        Ab[p][q2] = r <= cc ? A(r, cc) : A(cc, r);
+                    }
+                double n3[3] = {nrm[c][0], nrm[c][1], nrm[c][2]}, t1[3], t2[3];
+                nt_basis_d(n3, t1, t2);
+                const double* rows[3] = {n3, t1, t2};
+                double out[3] = {0, 0, 0};
+                for (int b = 0; b < 3; ++b) {
+                    const double* e = rows[b];
+                    double Ae[3];
+                    for (int p = 0; p < 3; ++p) Ae[p] = Ab[p][0] * e[0] + Ab[p][1] * e[1] + Ab[p][2] * e[2];
+                    const double dgl = e[0] * Ae[0] + e[1] * Ae[1] + e[2] * Ae[2];
+                    const double ex = e[0] * xs[j] + e[1] * xs[6 + j] + e[2] * xs[12 + j];
+                    for (int p = 0; p < 3; ++p) out[p] += dgl * ex * e[p];
+                }
+                for (int p = 0; p < 3; ++p) ys[p * 6 + j] = (T)out[p];
+            }
+        } else {
+            T xm[18];
+#pragma unroll
+            for (int i = 0; i < 18; ++i) xm[i] = xs[i];
+            // trial side masks
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                const int mode = j < 3 ? bmode : tmode;
+                if (mode == TN_DIRICHLET) {
+                    xm[j] = xm[6 + j] = xm[12 + j] = T(0);
+                } else if (mode == TN_FREESLIP) {
+                    const int c = WCORN[w][j % 3];
+                    const T un = (T)nrm[c][0] * xm[j] + (T)nrm[c][1] * xm[6 + j] + (T)nrm[c][2] * xm[12 + j];
+                    for (int k = 0; k < 3; ++k) xm[k * 6 + j] -= un * (T)nrm[c][k];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 18; ++i) ys[i] = T(0);
+#pragma unroll
+            for (int i = 0; i < 18; ++i) {
+#pragma unroll
+                for (int j = i; j < 18; ++j) {
+                    const T aij = A(i, j);
+                    ys[i] += aij * xm[j];
+                    if (j != i) ys[j] += aij * xm[i];
+                }
+            }
+            // test side
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                const int mode = j < 3 ? bmode : tmode;
+                if (mode == TN_DIRICHLET) {
+                    for (int k = 0; k < 3; ++k) ys[k * 6 + j] = A(k * 6 + j, k * 6 + j) * xs[k * 6 + j];
+                } else if (mode == TN_FREESLIP) {
+                    const int c = WCORN[w][j % 3];
+                    T nA[3];  // A_bb n
+                    for (int p = 0; p < 3; ++p) {
+                        T s = 0;
+                        for (int q2 = 0; q2 < 3; ++q2) {
+                            const int r = p * 6 + j, cc = q2 * 6 + j;
+                            s += (r <= cc ? A(r, cc) : A(cc, r)) * (T)nrm[c][q2];
+                        }
+                        nA[p] = s;
+                    }
+                    const T ann = (T)nrm[c][0] * nA[0] + (T)nrm[c][1] * nA[1] + (T)nrm[c][2] * nA[2];
+                    const T un = (T)nrm[c][0] * xs[j] + (T)nrm[c][1] * xs[6 + j] + (T)nrm[c][2] * xs[12 + j];
+                    const T yn = (T)nrm[c][0] * ys[j] + (T)nrm[c][1] * ys[6 + j] + (T)nrm[c][2] * ys[12 + j];
+                    for (int k = 0; k < 3; ++k) ys[k * 6 + j] += (ann * un - yn) * (T)nrm[c][k];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int c = WCORN[w][j % 3];
+            for (int k = 0; k < 3; ++k) {
+                if (j < 3)
+                    yb[c][k] += ys[k * 6 + j];
+                else
+                    yt[c][k] += ys[k * 6 + j];
+            }
+        }
+    }
+    // radial merge with the previous lane, then scatter
+    const int lane = threadIdx.x & 63;
+    const bool take = active && lane > 0 && rc > 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const T v = __shfl_up(yt[c][k], 1, 64);
+            if (take) yb[c][k] += v;
+        }
+    if (!active) return;
+    const bool top_flush = lane == 63 || rc == ncr - 1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int64_t ib = latc[c] * a.nr + rc;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            unsafeAtomicAdd(a.dst + k * N + ib, yb[c][k]);
+            if (top_flush) unsafeAtomicAdd(a.dst + k * N + ib + 1, yt[c][k]);
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host launchers
+template <typename T>
+void launch_prolongate_add(const MeshLevel& mc, const MeshLevel& mf, int ncomp, const T* xc, T* xf, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_prolongate_add<T>, dim3((unsigned)((mf.N + 255) / 256)), dim3(256), 0, s, xc, xf, mf.n, mf.nr,
+                       mc.n, mc.nr, mf.N, mc.N, ncomp);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_restrict(const MeshLevel& mf, const MeshLevel& mc, int ncomp, const T* xf, T* xc, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((mc.N + 255) / 256)), dim3(256), 0, s, xf, xc,
+                       mf.d_owned_lat.p, mf.n, mf.nr, mc.n, mc.nr, mf.N, mc.N, ncomp);
+    TN_HIP(hipGetLastError());
+    launch_interface_sum<T>(mc, ncomp, xc, s);
+}
+
+template <typename T, typename S, typename SF>
+void launch_gca_build(const GcaArgs<T, S, SF>& a, bool fine_geo, hipStream_t s)
+{
+    const int64_t ncc = 10LL * (a.nc - 1) * (a.nc - 1) * (a.nrc - 1);
+    const int64_t total = ncc * 12;
+    const dim3 gr((unsigned)((total + 127) / 128)), bl(128);
+    if (fine_geo)
+        hipLaunchKernelGGL((k_gca_build<T, S, SF, true>), gr, bl, 0, s, a);
+    else
+        hipLaunchKernelGGL((k_gca_build<T, S, SF, false>), gr, bl, 0, s, a);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T, typename S>
+void launch_stored_apply(const StoredArgs<T, S>& a, bool diagonal, hipStream_t s)
+{
+    const int64_t ncell = 10LL * (a.n - 1) * (a.n - 1) * (a.nr - 1);
+    const dim3 gr((unsigned)((ncell + 255) / 256)), bl(256);
+    if (diagonal)
+        hipLaunchKernelGGL((k_stored_apply<T, S, true>), gr, bl, 0, s, a);
+    else
+        hipLaunchKernelGGL((k_stored_apply<T, S, false>), gr, bl, 0, s, a);
+    TN_HIP(hipGetLastError());
+}
+
+#define TN_INST_T(T)                                                                                           \
+    template void launch_prolongate_add<T>(const MeshLevel&, const MeshLevel&, int, const T*, T*, hipStream_t); \
+    template void launch_restrict<T>(const MeshLevel&, const MeshLevel&, int, const T*, T*, hipStream_t);
+TN_INST_T(double)
+TN_INST_T(float)
+#define TN_INST_GCA(T, S, SF) template void launch_gca_build<T, S, SF>(const GcaArgs<T, S, SF>&, bool, hipStream_t);
+TN_INST_GCA(double, double, double)
+TN_INST_GCA(double, double, float)
+TN_INST_GCA(double, float, double)
+TN_INST_GCA(double, float, float)
+TN_INST_GCA(float, double, double)
+TN_INST_GCA(float, double, float)
+TN_INST_GCA(float, float, double)
+TN_INST_GCA(float, float, float)
+#define TN_INST_ST(T, S) template void launch_stored_apply<T, S>(const StoredArgs<T, S>&, bool, hipStream_t);
+TN_INST_ST(double, double)
+TN_INST_ST(double, float)
+TN_INST_ST(float, float)
+TN_INST_ST(float, double)
+
+}  // namespace tn

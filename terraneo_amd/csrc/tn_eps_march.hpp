@@ -1,0 +1,37 @@
+// Atomic-free epsilon-div-div apply (lateral march), launch interface.
+#pragma once
+
+#include "tn_internal.hpp"
+
+namespace tn {
+
+constexpr int MARCH_WAVES = 8;  // waves per block; MARCH_WAVES-1 node rows owned per block
+
+template <typename T>
+struct MarchArgs {
+    const T* src;             // [3][N]
+    T* dst;                   // [3][N]
+    const T* eta;             // [N]
+    const T* geo;             // [10][n-1][n-1][2][GEO_STRIDE]
+    const T* coords;          // [10][n][n][3] unit lateral coordinates
+    const T* radii;           // [nr]
+    const double* coords_d;   // FP64 copies for the rotation penalty
+    const double* radii_d;
+    const T* blk;             // [2][10 n n][3] boundary quadratic forms, or null (both shells Neumann)
+    const double* pen;        // fs/fs penalty: [0..3) beta, [4..13) S, or null
+    int n, nr;
+    int seg;                  // node columns per block in the march direction
+    int bc_cmb, bc_surf;
+    int add;                  // 1: dst += A x, 0: dst = A x
+};
+
+bool march_supported(int nr);
+
+// boundary-node quadratic forms e^T B e (see k_boundary_blocks)
+template <typename T>
+void launch_boundary_blocks(const Geom<T>& g, const T* eta, int bc_cmb, int bc_surf, T* blk, hipStream_t s);
+
+template <typename T>
+void launch_eps_march(const MarchArgs<T>& a, bool diagonal, hipStream_t s);
+
+}  // namespace tn

@@ -141,6 +141,27 @@ template <typename T>
 void launch_penalty_init(const MeshLevel& m, const double* beta_dev, const double* S_dev, T* y, bool add,
                          hipStream_t s);
 
+template <typename T>
+struct MultiDotArgs {
+    static constexpr int MAX = 16;
+    const T* b[MAX];
+    int nb;
+};
+
+template <typename T>
+void launch_mul(T* y, const T* d, int64_t count, hipStream_t s);
+template <typename T>
+void launch_invert(T* y, int64_t count, hipStream_t s);
+template <typename TD, typename TS>
+void launch_cast(TD* y, const TS* x, int64_t count, hipStream_t s);
+template <typename T>
+void launch_randomize(T* y, int64_t count, uint64_t seed, hipStream_t s);
+template <typename T>
+void launch_nonfinite(const T* y, int64_t count, int* flag_dev, hipStream_t s);
+template <typename T>
+void launch_multidot(const MeshLevel& m, int ncomp, const T* a, const MultiDotArgs<T>& bs, double* out_dev,
+                     hipStream_t s);
+
 }  // namespace tn
 
 struct tn_mesh {

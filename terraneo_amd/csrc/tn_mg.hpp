@@ -1,0 +1,70 @@
+// Argument blocks of the multigrid kernels and their host launchers.
+#pragma once
+
+#include "tn_internal.hpp"
+
+namespace tn {
+
+// normal-tangential basis (rows n, t1, t2), double precision: the reference
+// axis is the coordinate axis least aligned with n
+// (local_basis_trafo_normal_tangential.hpp:34-79)
+__device__ inline void nt_basis_d(const double n[3], double t1[3], double t2[3])
+{
+    const double ax = fabs(n[0]), ay = fabs(n[1]), az = fabs(n[2]);
+    double ref[3] = {0, 0, 0};
+    if (ax <= ay && ax <= az)
+        ref[0] = 1;
+    else if (ay <= ax && ay <= az)
+        ref[1] = 1;
+    else
+        ref[2] = 1;
+    t1[0] = ref[1] * n[2] - ref[2] * n[1];
+    t1[1] = ref[2] * n[0] - ref[0] * n[2];
+    t1[2] = ref[0] * n[1] - ref[1] * n[0];
+    double l = sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+    if (l < 1e-15) {
+        t1[0] = n[1];
+        t1[1] = -n[0];
+        t1[2] = 0;
+        l = sqrt(t1[0] * t1[0] + t1[1] * t1[1]);
+    }
+    for (int k = 0; k < 3; ++k) t1[k] /= l;
+    t2[0] = n[1] * t1[2] - n[2] * t1[1];
+    t2[1] = n[2] * t1[0] - n[0] * t1[2];
+    t2[2] = n[0] * t1[1] - n[1] * t1[0];
+    l = sqrt(t2[0] * t2[0] + t2[1] * t2[1] + t2[2] * t2[2]);
+    for (int k = 0; k < 3; ++k) t2[k] /= l;
+}
+
+template <typename T, typename S>
+struct StoredArgs {
+    const S* stored;       // [2*171][cells]
+    const T* src;
+    T* dst;
+    const double* coords;  // [10][n][n][3]
+    int n, nr;
+    int bc_cmb, bc_surf;
+};
+
+template <typename T, typename S, typename SF>
+struct GcaArgs {
+    int nc, nrc, nf, nrf;
+    const double* geo_f;    // fine geometry table (fine level matrix-free)
+    const double* radii_f;
+    const T* eta_f;         // fine nodal viscosity
+    const SF* stored_f; // fine packed storage (fine level stored)
+    S* stored_c;        // coarse packed storage (output)
+};
+
+constexpr int PACKED = 171;
+
+template <typename T>
+void launch_prolongate_add(const MeshLevel& mc, const MeshLevel& mf, int ncomp, const T* xc, T* xf, hipStream_t s);
+template <typename T>
+void launch_restrict(const MeshLevel& mf, const MeshLevel& mc, int ncomp, const T* xf, T* xc, hipStream_t s);
+template <typename T, typename S, typename SF>
+void launch_gca_build(const GcaArgs<T, S, SF>& a, bool fine_geo, hipStream_t s);
+template <typename T, typename S>
+void launch_stored_apply(const StoredArgs<T, S>& a, bool diagonal, hipStream_t s);
+
+}  // namespace tn

@@ -1,0 +1,239 @@
+// Host-side linear algebra over device fields: block vectors, level-1
+// operations, operator/solver interfaces and the Krylov / multigrid drivers.
+//
+// Mirrors the reference's concept layer (linalg/operator.hpp:56-159,
+// linalg/vector.hpp:8-170, linalg/solvers/solver.hpp:19-54) with runtime
+// polymorphism: the solvers are host code, every vector operation is a HIP
+// kernel on the context's stream, and every scalar that a recurrence needs
+// on the host (dots, norms) is one device reduction plus one 8-byte copy.
+#pragma once
+
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "tn_internal.hpp"
+
+namespace tn {
+
+struct Field {
+    void* p = nullptr;
+    int dtype = TN_F64;
+    int ncomp = 3;
+    MeshLevel* m = nullptr;
+    int64_t count() const { return m->N * (int64_t)ncomp; }
+    size_t esize() const { return dtype == TN_F64 ? 8 : 4; }
+};
+
+// One block (a velocity or scalar field) or two (Stokes velocity + pressure).
+struct BVec {
+    Field b[2];
+    int nb = 1;
+    Field& u() { return b[0]; }
+    Field& p() { return b[1]; }
+    const Field& u() const { return b[0]; }
+    const Field& p() const { return b[1]; }
+};
+
+// Device memory owner for vectors shaped like a template.
+class VecStore {
+  public:
+    BVec make(const BVec& like);
+    BVec make_like(const BVec& like, int dtype);
+    ~VecStore();
+
+  private:
+    std::vector<void*> ptrs_;
+};
+
+struct Ctx {
+    hipStream_t s = nullptr;
+    DevBuf<double> red;  // reduction slots
+    DevBuf<int> flag;
+    Ctx();
+    double fetch(int slots, double* out = nullptr);
+};
+
+// level-1 (all blocks)
+void v_fill(Ctx& c, const BVec& y, double v);
+void v_lincomb(Ctx& c, const BVec& y, std::initializer_list<double> coef, std::initializer_list<const BVec*> xs,
+               double c0 = 0.0);
+void v_assign(Ctx& c, const BVec& y, const BVec& x);
+double v_dot(Ctx& c, const BVec& a, const BVec& b);
+void v_mul(Ctx& c, const BVec& y, const BVec& d);
+void v_invert(Ctx& c, const BVec& y);
+void v_randomize(Ctx& c, const BVec& y, uint64_t seed);
+bool v_nonfinite(Ctx& c, const BVec& y);
+void v_convert(Ctx& c, const BVec& dst, const BVec& src);
+// out[k] = a . bs[k] (k < n <= 16), one pass over a
+void v_multidot(Ctx& c, const BVec& a, const std::vector<const BVec*>& bs, double* out);
+
+struct LinOp {
+    virtual void apply(Ctx& c, const BVec& x, const BVec& y) = 0;
+    virtual ~LinOp() = default;
+};
+
+struct Solver {
+    // x is the initial guess on entry (reference semantics: solvers start from x)
+    virtual void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) = 0;
+    virtual ~Solver() = default;
+};
+
+struct IterInfo {
+    int iterations = 0;
+    double initial_residual = 0.0;
+    double final_residual = 0.0;
+    std::vector<double> history;  // absolute residual per iteration (incl. 0)
+};
+
+struct KrylovParams {
+    int max_iterations = 100;
+    double rtol = 1e-8;
+    double atol = 1e-30;
+    int restart = 30;
+};
+
+// ---- solvers (reference files cited in solvers.cpp)
+class IdentitySolver : public Solver {
+  public:
+    void solve(Ctx& c, LinOp&, const BVec& x, const BVec& b) override { v_assign(c, x, b); }
+};
+
+class DiagonalSolver : public Solver {  // x = b * inv_diag
+  public:
+    explicit DiagonalSolver(BVec inv_diag) : d_(inv_diag) {}
+    void solve(Ctx& c, LinOp&, const BVec& x, const BVec& b) override;
+
+  private:
+    BVec d_;
+};
+
+class Chebyshev : public Solver {
+  public:
+    Chebyshev(int order, int iterations, BVec inv_diag, BVec t0, BVec t1, int power_iterations, uint64_t seed);
+    void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+    double lambda_max() const { return lambda_; }
+    void set_lambda_max(double l)
+    {
+        lambda_ = l;
+        need_lambda_ = false;
+    }
+
+  private:
+    int order_, iters_, power_its_;
+    uint64_t seed_;
+    BVec dinv_, d_, z_;
+    bool need_lambda_ = true;
+    double lambda_ = 0.0;
+};
+
+class PCG : public Solver {
+  public:
+    PCG(KrylovParams p, std::vector<BVec> tmps, Solver* prec);
+    void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+    IterInfo info;
+
+  private:
+    KrylovParams p_;
+    std::vector<BVec> t_;
+    Solver* prec_;
+    IdentitySolver id_;
+};
+
+class PMINRES : public Solver {
+  public:
+    PMINRES(KrylovParams p, std::vector<BVec> tmps, Solver* prec);
+    void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+    IterInfo info;
+
+  private:
+    KrylovParams p_;
+    std::vector<BVec> t_;
+    Solver* prec_;
+    IdentitySolver id_;
+};
+
+class FGMRES : public Solver {
+  public:
+    FGMRES(KrylovParams p, std::vector<BVec> tmps, Solver* prec, bool skip_nonfinite_prec = true);
+    void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+    IterInfo info;
+    int restarts_breakdown = 0, restarts_stagnation = 0, prec_skipped = 0;
+
+  private:
+    KrylovParams p_;
+    std::vector<BVec> t_;
+    Solver* prec_;
+    IdentitySolver id_;
+    bool skip_nonfinite_;
+};
+
+// One multigrid level pair: transfers between level l (coarse) and l+1.
+struct Transfer {
+    MeshLevel* coarse;
+    MeshLevel* fine;
+};
+
+class Multigrid : public Solver {
+  public:
+    // ops[l], smoothers[l] for l = 0..L (0 coarsest, L finest operator = the
+    // operator passed to solve); coarse solver acts on level 0.
+    Multigrid(std::vector<LinOp*> coarse_ops, std::vector<Transfer> transfers, std::vector<Solver*> smoothers,
+              Solver* coarse, std::vector<BVec> tmp, std::vector<BVec> r, std::vector<BVec> e, int cycles);
+    void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+
+  private:
+    void recurse(Ctx& c, LinOp& A, const BVec& x, const BVec& b, int level);
+    std::vector<LinOp*> ops_;
+    std::vector<Transfer> tr_;
+    std::vector<Solver*> sm_;
+    Solver* coarse_;
+    std::vector<BVec> tmp_, r_, e_;
+    int cycles_;
+};
+
+// 2x2 block preconditioners on Stokes vectors (block 0 velocity, 1 pressure)
+class BlockDiagonal : public Solver {
+  public:
+    BlockDiagonal(LinOp* A11, Solver* p11, LinOp* A22, Solver* p22) : A11_(A11), p11_(p11), A22_(A22), p22_(p22) {}
+    void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+
+  private:
+    LinOp *A11_;
+    Solver* p11_;
+    LinOp* A22_;
+    Solver* p22_;
+};
+
+class BlockTriangular : public Solver {
+  public:
+    BlockTriangular(LinOp* A11, Solver* p11, LinOp* A22, Solver* p22, LinOp* B12, BVec tmp)
+        : A11_(A11), p11_(p11), A22_(A22), p22_(p22), B12_(B12), tmp_(tmp)
+    {
+    }
+    void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+
+  private:
+    LinOp* A11_;
+    Solver* p11_;
+    LinOp* A22_;
+    Solver* p22_;
+    LinOp* B12_;
+    BVec tmp_;
+};
+
+// Runs an inner solver in another precision: x, b are converted to the inner
+// dtype, the inner solver starts from the converted x, the result is
+// converted back.
+class PrecisionBridge : public Solver {
+  public:
+    PrecisionBridge(Solver* inner, LinOp* inner_op, BVec xi, BVec bi) : inner_(inner), op_(inner_op), xi_(xi), bi_(bi) {}
+    void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+
+  private:
+    Solver* inner_;
+    LinOp* op_;
+    BVec xi_, bi_;
+};
+
+}  // namespace tn

@@ -1,0 +1,45 @@
+// Argument blocks of the Stokes coupling / mass kernels.
+#pragma once
+
+#include "tn_internal.hpp"
+
+namespace tn {
+
+template <typename T>
+struct DivArgs {
+    const double* coords;  // fine lateral coords
+    const double* radii;   // fine radii
+    int nf, nrf, nc, nrc;
+    int bc_cmb, bc_surf;
+    const T* u;  // fine velocity [3][Nf]
+    T* p;        // coarse pressure [Nc] (accumulated)
+};
+
+template <typename T>
+struct GradArgs {
+    const double* coords;
+    const double* radii;
+    int nf, nrf, nc, nrc;
+    int bc_cmb, bc_surf;
+    const T* p;  // coarse pressure
+    T* u;        // fine velocity (accumulated)
+};
+
+template <typename T>
+struct MassArgs {
+    const double* coords;
+    const double* radii;
+    int n, nr, ncomp;
+    const T* x;  // input (vector mass)
+    const T* k;  // coefficient (lumped k-mass)
+    T* y;        // accumulated output
+};
+
+template <typename T>
+void launch_div(const DivArgs<T>& a, hipStream_t s);
+template <typename T>
+void launch_grad(const GradArgs<T>& a, hipStream_t s);
+template <typename T>
+void launch_mass(const MassArgs<T>& a, bool lumped_k, hipStream_t s);
+
+}  // namespace tn

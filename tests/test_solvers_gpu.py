@@ -1,0 +1,90 @@
+"""GPU solvers: PCG recurrence vs a numpy restatement over the oracle, and the
+epsilon-operator problem of the reference's CPU configuration (lateral 4 x 17
+radial, eta = exp(ln100 (r-0.55)/0.45), zero Dirichlet, 3-level GMG with
+Chebyshev degree 2, GCA coarse operators, PCG to 1e-10) against a converged
+CPU solve of the oracle's assembled matrix.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from fields import consistent_random, radial_eta, smooth_eta
+
+
+def _rhs(om, seed):
+    """iid U[-1,1] per owned node and component, copied to duplicates, zero on both shells."""
+    b = consistent_random(om, 3, seed).reshape(3, 10, om.n, om.n, om.nr)
+    b[..., 0] = 0.0
+    b[..., -1] = 0.0
+    return np.ascontiguousarray(b.reshape(-1))
+
+
+@pytest.mark.gpu
+def test_pcg_history_matches_numpy_restatement(cuda):
+    import torch
+
+    import terraneo_amd as tn
+
+    m = tn.Mesh(3, 3, 0.55, 1.0)
+    om = O.Mesh(3, m.nr, 0.55, 1.0)
+    eta = smooth_eta(om.coords, om.radii).reshape(-1)
+    A = tn.Eps(m, torch.from_numpy(eta).to(cuda), tn.TN_DIRICHLET, tn.TN_DIRICHLET)
+    b = _rhs(om, 3)
+    x = torch.zeros(3 * m.N, dtype=torch.float64, device=cuda)
+    info = tn.pcg_solve(A, x, torch.from_numpy(b).to(cuda), max_iterations=25, rtol=1e-30)
+    # numpy PCG (identity preconditioner) over the oracle operator, pcg.hpp:78-153
+    oA = O.Eps(om, eta, O.DIRICHLET, O.DIRICHLET)
+    xs = np.zeros_like(b)
+    r = b - oA.apply(xs)
+    z = r.copy()
+    p = z.copy()
+    hist = [np.sqrt(om.dot(r, r, 3))]
+    for _ in range(25):
+        an = om.dot(z, r, 3)
+        ap = oA.apply(p)
+        al = an / om.dot(ap, p, 3)
+        xs += al * p
+        r -= al * ap
+        hist.append(np.sqrt(om.dot(r, r, 3)))
+        z = r.copy()
+        p = z + om.dot(z, r, 3) / an * p
+    h = info["history"]
+    assert len(h) == 26
+    np.testing.assert_allclose(h, hist, rtol=1e-8)
+    # 25 unconverged CG steps amplify rounding differences in x; the residual
+    # history above is the recurrence check
+    np.testing.assert_allclose(x.cpu().numpy(), xs, rtol=0, atol=1e-6 * np.abs(xs).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("store", ["f64", "f32"])
+def test_c0_pcg_gmg_gca_solves(cuda, store):
+    import scipy.sparse.linalg as spl
+    import torch
+
+    import terraneo_amd as tn
+
+    lat, rad = 4, 4
+    meshes = [tn.Mesh(l, r, 0.55, 1.0) for l, r in [(2, 2), (3, 3), (4, 4)]]
+    oms = [O.Mesh(l, m.nr, 0.55, 1.0) for l, m in zip((2, 3, 4), meshes)]
+    etas = [radial_eta(om.radii, (10, om.n, om.n, om.nr)).reshape(-1) for om in oms]
+    ops = [tn.Eps(m, torch.from_numpy(e).to(cuda), tn.TN_DIRICHLET, tn.TN_DIRICHLET) for m, e in zip(meshes, etas)]
+    sd = tn.TN_F64 if store == "f64" else tn.TN_F32
+    ops[1].gca_from(ops[2], sd)
+    ops[0].gca_from(ops[1], sd)
+    mg = tn.Gmg(ops, tn.api.gmg_options(cheby_order=2, cheby_iterations=3, power_iterations=10, num_cycles=1))
+    om = oms[-1]
+    b = _rhs(om, 0)
+    x = torch.zeros(3 * meshes[-1].N, dtype=torch.float64, device=cuda)
+    info = tn.pcg_solve(ops[-1], x, torch.from_numpy(b).to(cuda), prec=mg, max_iterations=100, rtol=1e-10)
+    print("C0 PCG+GMG", store, info["iterations"], "its", info["seconds"], "s")
+    assert info["final_residual"] / info["initial_residual"] <= 1e-10
+    assert info["iterations"] < 40
+    # CPU: converged CG on the oracle's assembled Dirichlet operator (unique dofs)
+    oA = O.eps_assemble(O.Eps(om, etas[-1], O.DIRICHLET, O.DIRICHLET))
+    bu = O.to_unique(om, b, 3)
+    d = oA.diagonal()
+    ref, ok = spl.cg(oA, bu, rtol=1e-13, maxiter=20000, M=spl.LinearOperator(oA.shape, matvec=lambda v: v / d))
+    assert ok == 0
+    xu = O.to_unique(om, x.cpu().numpy(), 3)
+    assert np.abs(xu - ref).max() <= 1e-8 * np.abs(ref).max()
