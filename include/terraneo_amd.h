@@ -156,6 +156,18 @@ int tn_eps_gca_from(tn_eps* coarse, const tn_eps* fine, int store_dtype, tn_stre
  * dimension-wise dof order (component*6 + wedge node); synchronous */
 int tn_eps_get_local_matrix(const tn_eps* op, int sd, int x, int y, int r, int wedge, double* out);
 
+/* ------------------------------------------------------------------ adjoint viscosity gradient
+ * No counterpart in the reference (its viscosity is nodal, kerngen.hpp:94-96).
+ * Per-hex-cell viscosity eta_cell [10][n-1][n-1][nr-1] (device, operator dtype):
+ * both wedges of a cell use kwJ = eta_cell |det J|; overrides the nodal
+ * viscosity; NULL returns to it.  Same caching rule as tn_eps_set_eta. */
+int tn_eps_set_cell_eta(tn_eps* op, const void* eta_cell);
+/* Backward half of the per-cell log-viscosity parameterisation m = ln eta_cell:
+ *   grad[cell] = -eta_cell[cell] (T lambda)_cell^T A_cell(1) u_cell
+ * (u: converged velocity, lambda: velocity part of the adjoint solution,
+ * T: the boundary shells' trial rule).  grad: [10][n-1][n-1][nr-1]. */
+int tn_eps_grad_log_cell_eta(tn_eps* op, const void* u, const void* lambda, void* grad, tn_stream_t s);
+
 /* ------------------------------------------------------------------ Stokes block
  * EpsDivDivStokes (epsilon_divdiv_stokes.hpp:19-130): [A B^T; B 0] with the
  * Q1-iso-Q2 velocity on `vel` and Q1 pressure on `pre` (one level coarser).
@@ -188,7 +200,9 @@ int tn_freeslip_project(const tn_mesh* m, int dtype, void* v, int which, tn_stre
 /* ------------------------------------------------------------------ solvers
  * Multigrid with Chebyshev smoothing over `nlevels` operators ordered coarse
  * to fine (Multigrid multigrid.hpp:24-212, Chebyshev chebyshev.hpp:50-177).
- * The operators are referenced, not owned. */
+ * The operators are referenced, not owned.  All levels share the finest
+ * level's dtype, except the coarsest, which may be TN_F64 under a TN_F32
+ * hierarchy (its PCG solve then runs in FP64 behind a precision bridge). */
 int tn_gmg_create(int nlevels, tn_eps* const* ops, const tn_gmg_options* opts, tn_stream_t s, tn_gmg** out);
 int tn_gmg_destroy(tn_gmg* g);
 /* x <- V-cycle(s) for A_finest x = b, starting from x */

@@ -450,6 +450,17 @@ static void trafo_nt(const double nin[3], double R[3][3])
 void orc_eps_local_matrix(const orc_mesh* m, const double* k, int d, int x, int y, int r, int wedge, double* A)
 {
     const int n = m->n, nr = m->nr;
+    const double q[3] = {1.0 / 3.0, 1.0 / 3.0, 0.0};
+    double keval = 0.0;
+    for (int j = 0; j < 6; j++)
+        keval += shape(j, q) * k[IDX4(n, nr, d, x + WOFF[wedge][j][0], y + WOFF[wedge][j][1], r + WOFF[wedge][j][2])];
+    orc_eps_local_matrix_k(m, keval, d, x, y, r, wedge, A);
+}
+
+/* assemble_local_matrix, epsilon_divdiv_kerngen.hpp:2051-2108, with the
+ * quadrature-point viscosity keval given */
+void orc_eps_local_matrix_k(const orc_mesh* m, double keval, int d, int x, int y, int r, int wedge, double* A)
+{
     double P[2][3][3];
     wedge_surf(m, d, x, y, P);
     const double q[3] = {1.0 / 3.0, 1.0 / 3.0, 0.0};
@@ -458,9 +469,6 @@ void orc_eps_local_matrix(const orc_mesh* m, const double* k, int d, int x, int 
     jac(P[wedge], r1, r2, q, J);
     const double det = det3(J);
     inv_t3(J, det, K);
-    double keval = 0.0;
-    for (int j = 0; j < 6; j++)
-        keval += shape(j, q) * k[IDX4(n, nr, d, x + WOFF[wedge][j][0], y + WOFF[wedge][j][1], r + WOFF[wedge][j][2])];
     const double w = 1.0 * keval * fabs(det);
     double g[6][3];
     for (int j = 0; j < 6; j++) {
@@ -570,6 +578,9 @@ static void get_local(const orc_eps* op, int d, int x, int y, int r, int w, doub
     if (op->stored) {
         const int64_t c = (((int64_t)d * (m->n - 1) + x) * (m->n - 1) + y) * (m->nr - 1) + r;
         memcpy(A, op->stored + (c * 2 + w) * 324, sizeof(double) * 324);
+    } else if (op->kc) {
+        const int64_t c = (((int64_t)d * (m->n - 1) + x) * (m->n - 1) + y) * (m->nr - 1) + r;
+        orc_eps_local_matrix_k(m, op->kc[c], d, x, y, r, w, A);
     } else {
         orc_eps_local_matrix(m, op->k, d, x, y, r, w, A);
     }
@@ -1223,4 +1234,53 @@ void orc_gca_build(const orc_eps* fine, const orc_mesh* mc, double* out)
                     memcpy(out + (c * 2) * 324, Ac[0], sizeof(double) * 324);
                     memcpy(out + (c * 2 + 1) * 324, Ac[1], sizeof(double) * 324);
                 }
+}
+
+/* ---------------------------------------------------------------- cell viscosity + adjoint gradient */
+void orc_eps_set_cell_viscosity(orc_eps* op, const double* kc) { op->kc = kc; }
+
+void orc_eps_grad_log_cell_viscosity(const orc_eps* op, const double* u, const double* lam, double* g)
+{
+    const orc_mesh* m = op->m;
+    const int n = m->n, nr = m->nr, ncl = n - 1, ncr = nr - 1;
+    const int64_t N = orc_mesh_nodes(m);
+    const int64_t cells = 10LL * ncl * ncl * ncr;
+#pragma omp parallel for schedule(static)
+    for (int64_t cell = 0; cell < cells; cell++) {
+        const int r = (int)(cell % ncr);
+        const int y = (int)((cell / ncr) % ncl);
+        const int x = (int)((cell / ((int64_t)ncr * ncl)) % ncl);
+        const int d = (int)(cell / ((int64_t)ncr * ncl * ncl));
+        double acc = 0.0;
+        for (int w = 0; w < 2; w++) {
+            double A[324], ue[18], le[18];
+            orc_eps_local_matrix_k(m, 1.0, d, x, y, r, w, A);
+            for (int j = 0; j < 6; j++) {
+                const int gx = x + WOFF[w][j][0], gy = y + WOFF[w][j][1], gr = r + WOFF[w][j][2];
+                const int64_t idx = IDX4(n, nr, d, gx, gy, gr);
+                double lv[3];
+                for (int c = 0; c < 3; c++) {
+                    ue[j + 6 * c] = u[c * N + idx];
+                    lv[c] = lam[c * N + idx];
+                }
+                const int mode = gr == 0 ? op->bc_cmb : (gr == nr - 1 ? op->bc_surf : ORC_NEUMANN);
+                if (mode == ORC_DIRICHLET) {
+                    lv[0] = lv[1] = lv[2] = 0.0;
+                } else if (mode == ORC_FREESLIP) {
+                    const double* cc = m->coords + 3 * (((int64_t)d * n + gx) * n + gy);
+                    const double nn = sqrt(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
+                    const double nv[3] = {cc[0] / nn, cc[1] / nn, cc[2] / nn};
+                    const double ln = nv[0] * lv[0] + nv[1] * lv[1] + nv[2] * lv[2];
+                    for (int c = 0; c < 3; c++) lv[c] -= ln * nv[c];
+                }
+                for (int c = 0; c < 3; c++) le[j + 6 * c] = lv[c];
+            }
+            for (int i = 0; i < 18; i++) {
+                double s = 0.0;
+                for (int j = 0; j < 18; j++) s += A[i * 18 + j] * ue[j];
+                acc += le[i] * s;
+            }
+        }
+        g[cell] = -op->kc[cell] * acc;
+    }
 }

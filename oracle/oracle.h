@@ -53,10 +53,13 @@ int64_t   orc_mesh_nodes(const orc_mesh* m);
 /* raw wedge-local 18x18 matrix (dimension-wise dof order d*6+node), no BC */
 void orc_eps_local_matrix(const orc_mesh* m, const double* k, int d, int x, int y, int r,
                           int wedge, double* A);
+/* same with the quadrature-point viscosity given directly (keval) */
+void orc_eps_local_matrix_k(const orc_mesh* m, double keval, int d, int x, int y, int r, int wedge, double* A);
 
 typedef struct orc_eps {
     const orc_mesh* m;
     const double* k;      /* nodal viscosity [10][n][n][nr] */
+    const double* kc;     /* optional per hex cell viscosity [10][n-1][n-1][nr-1]; overrides k */
     int bc_cmb, bc_surf;
     double* stored;       /* optional [10][n-1][n-1][nr-1][2][18*18] (GCA) */
     double* null_modes;   /* fs/fs penalty modes [3][3*N] */
@@ -75,6 +78,12 @@ void     orc_eps_cell_effective(const orc_eps* op, int d, int x, int y, int r, i
 int64_t  orc_eps_assemble_coo(const orc_eps* op, int64_t* rows, int64_t* cols, double* vals);
 /* gather the fs/fs penalty modes (each 3*N doubles); returns 0 when inactive */
 int      orc_eps_penalty_modes(const orc_eps* op, double* out);
+/* per hex cell viscosity (both wedges): kwJ = kc[cell] |det J| */
+void     orc_eps_set_cell_viscosity(orc_eps* op, const double* kc);
+/* adjoint gradient w.r.t. per-cell log-viscosity m = ln kc:
+ *   g[cell] = -kc[cell] sum_wedges (T lam)_e^T A_e(1) u_e
+ * T: trial rule of the boundary shells (Dirichlet: zero, free-slip: project) */
+void     orc_eps_grad_log_cell_viscosity(const orc_eps* op, const double* u, const double* lam, double* g);
 
 /* ---------------------------------------------------------------- Stokes coupling */
 /* B: fine velocity -> coarse pressure (coarse mesh level one lower) */

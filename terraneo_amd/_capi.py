@@ -72,6 +72,8 @@ SIGNATURES = {
     "tn_eps_set_penalty_epsilon": (i32, [vp, f64]),
     "tn_eps_gca_from": (i32, [vp, vp, i32, vp]),
     "tn_eps_get_local_matrix": (i32, [vp, i32, i32, i32, i32, i32, vp]),
+    "tn_eps_set_cell_eta": (i32, [vp, vp]),
+    "tn_eps_grad_log_cell_eta": (i32, [vp, vp, vp, vp, vp]),
     "tn_stokes_create": (i32, [vp, vp, vp, pvp]),
     "tn_stokes_destroy": (i32, [vp]),
     "tn_stokes_apply": (i32, [vp, vp, vp, vp, vp, vp]),

@@ -106,13 +106,29 @@ class Mesh:
 class Eps:
     """epsilon-div-div operator (EpsilonDivDivKerngen drop-in)."""
 
-    def __init__(self, mesh, eta, bc_cmb=TN_NEUMANN, bc_surf=TN_NEUMANN):
+    def __init__(self, mesh, eta, bc_cmb=TN_NEUMANN, bc_surf=TN_NEUMANN, eta_cell=None):
+        """eta: nodal viscosity [10][n][n][nr]; eta_cell (optional): per hex cell
+        viscosity [10][n-1][n-1][nr-1], which then replaces the nodal one."""
         self.mesh = mesh
         self.eta = eta  # keep alive: the operator holds a view
-        self.dtype = dtype_code(eta.dtype)
+        self.eta_cell = eta_cell
+        ref = eta if eta is not None else eta_cell
+        self.dtype = dtype_code(ref.dtype)
         h = C.c_void_p()
-        check(lib().tn_eps_create(mesh.h, self.dtype, _ptr(eta), bc_cmb, bc_surf, C.byref(h)))
+        check(lib().tn_eps_create(mesh.h, self.dtype, _ptr(eta) if eta is not None else None, bc_cmb, bc_surf,
+                                  C.byref(h)))
         self.h = h
+        if eta_cell is not None:
+            check(lib().tn_eps_set_cell_eta(self.h, _ptr(eta_cell)))
+
+    def set_cell_eta(self, eta_cell):
+        self.eta_cell = eta_cell
+        check(lib().tn_eps_set_cell_eta(self.h, _ptr(eta_cell) if eta_cell is not None else None))
+
+    def grad_log_cell_eta(self, u, lam, grad, stream=None):
+        """grad[cell] = -eta_cell (T lam)^T A_cell(1) u (adjoint viscosity gradient)."""
+        check(lib().tn_eps_grad_log_cell_eta(self.h, _ptr(u), _ptr(lam), _ptr(grad), _stream_ptr(stream)))
+        return grad
 
     def set_modes(self, apply_mode=TN_REPLACE, comm_mode=TN_COMM_ADDITIVE):
         check(lib().tn_eps_set_modes(self.h, apply_mode, comm_mode))

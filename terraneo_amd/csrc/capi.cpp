@@ -1,4 +1,5 @@
 // C ABI: mesh, level-1 operations and the epsilon-div-div operator.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -82,15 +83,44 @@ tn::Geom<float> geom_of<float>(const MeshLevel& m)
     return {m.d_coords_f.p, m.d_radii_f.p, m.d_geo_f.p, m.n, m.nr};
 }
 
-int march_seg(int n)
+int device_cus()
+{
+    static int cache[64] = {0};
+    int dev = 0;
+    TN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (!cache[dev]) {
+        int cus = 0;
+        TN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        cache[dev] = cus > 0 ? cus : 1;
+    }
+    return cache[dev];
+}
+
+// Segment length (node columns per block) of the march kernel: one block is
+// resident per CU, so the kernel takes ceil(blocks / CUs) rounds of
+// (seg + 1) quad columns each; pick the segment that minimises that product.
+// TN_MARCH_SEG overrides (tuning).
+int march_seg(int n, int dtype_size)
 {
     static const int env = [] {
         const char* e = std::getenv("TN_MARCH_SEG");
         return e ? std::atoi(e) : 0;
     }();
-    const int target = env > 0 ? env : 16;
-    const int nseg = (n + target - 1) / target;
-    return (n + nseg - 1) / nseg;
+    if (env > 0) return std::min(std::min(env, n), tn::MARCH_SEG_MAX);
+    const int own = tn::march_block_rows(dtype_size);
+    const int64_t nxb = (n + own - 1) / own, cus = device_cus();
+    int best = std::min(n, tn::MARCH_SEG_MAX);
+    int64_t best_cost = -1;
+    for (int seg = std::min(n, tn::MARCH_SEG_MAX); seg >= std::min(n, 6); --seg) {
+        const int64_t blocks = 10 * nxb * ((n + seg - 1) / seg);
+        const int64_t cost = ((blocks + cus - 1) / cus) * (seg + 1);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = seg;
+        }
+    }
+    return best;
 }
 
 template <typename T>
@@ -109,43 +139,35 @@ void eps_apply_t(tn_eps* op, const void* src_v, void* dst_v, hipStream_t s)
         tn::launch_fill<double>(op->pen.p, 4, 0.0, s);
         tn::launch_penalty_moment<T>(m, src, op->pen.p, s);
     }
-    if (op->stored_dtype < 0 && tn::march_supported(m.nr)) {
+    if (op->stored_dtype < 0) {
         // matrix-free, atomic-free: the kernel writes every node copy once
-        TN_REQUIRE(eta, "eps apply: viscosity not set");
+        TN_REQUIRE(eta || op->eta_cell, "eps apply: viscosity not set");
+        const T* eta_cell = static_cast<const T*>(op->eta_cell);
         const bool need_blk = op->bc_cmb != TN_NEUMANN || op->bc_surf != TN_NEUMANN;
         if (need_blk && !op->blk_ready) {
             const size_t bytes = sizeof(T) * 2 * 10 * (size_t)m.n * m.n * 3;
             if (op->blk.n != bytes) op->blk.alloc(bytes);
-            tn::launch_boundary_blocks<T>(geom_of<T>(m), eta, op->bc_cmb, op->bc_surf, (T*)op->blk.p, s);
+            tn::launch_boundary_blocks<T>(geom_of<T>(m), eta, eta_cell, op->bc_cmb, op->bc_surf, (T*)op->blk.p, s);
             op->blk_ready = true;
         }
         const tn::Geom<T> g = geom_of<T>(m);
-        tn::MarchArgs<T> a{src, dst, eta, g.geo, g.coords, g.radii, m.d_coords.p, m.d_radii.p,
+        tn::MarchArgs<T> a{src, dst, eta, eta_cell, g.geo, g.coords, g.radii, m.d_coords.p, m.d_radii.p,
                            need_blk ? (const T*)op->blk.p : nullptr, penalty ? op->pen.p : nullptr,
-                           m.n, m.nr, march_seg(m.n), op->bc_cmb, op->bc_surf, add ? 1 : 0};
+                           m.n, m.nr, march_seg(m.n, (int)sizeof(T)), op->bc_cmb, op->bc_surf, add ? 1 : 0};
         tn::launch_eps_march<T>(a, op->diagonal, s);
     } else {
         if (penalty)
             tn::launch_penalty_init<T>(m, op->pen.p, op->pen.p + 4, dst, add, s);
         else if (!add)
             tn::launch_fill<T>(dst, 3 * m.N, T(0), s);
-        if (op->stored_dtype >= 0) {
-            if (op->stored_dtype == TN_F64) {
-                tn::StoredArgs<T, double> a{(const double*)op->stored.p, src, dst, m.d_coords.p, m.n, m.nr,
-                                            op->bc_cmb, op->bc_surf};
-                tn::launch_stored_apply<T, double>(a, op->diagonal, s);
-            } else {
-                tn::StoredArgs<T, float> a{(const float*)op->stored.p, src, dst, m.d_coords.p, m.n, m.nr,
-                                           op->bc_cmb, op->bc_surf};
-                tn::launch_stored_apply<T, float>(a, op->diagonal, s);
-            }
+        if (op->stored_dtype == TN_F64) {
+            tn::StoredArgs<T, double> a{(const double*)op->stored.p, src, dst, m.d_coords.p, m.n, m.nr, op->bc_cmb,
+                                        op->bc_surf};
+            tn::launch_stored_apply<T, double>(a, op->diagonal, s);
         } else {
-            TN_REQUIRE(eta, "eps apply: viscosity not set");
-            tn::EpsLaunch cfg;
-            cfg.bc_cmb = op->bc_cmb;
-            cfg.bc_surf = op->bc_surf;
-            cfg.diagonal = op->diagonal;
-            tn::launch_eps_apply<T>(geom_of<T>(m), eta, src, dst, cfg, s);
+            tn::StoredArgs<T, float> a{(const float*)op->stored.p, src, dst, m.d_coords.p, m.n, m.nr, op->bc_cmb,
+                                       op->bc_surf};
+            tn::launch_stored_apply<T, float>(a, op->diagonal, s);
         }
     }
     if (op->comm_mode == TN_COMM_ADDITIVE) tn::launch_interface_sum<T>(m, 3, dst, s);
@@ -392,6 +414,33 @@ int tn_eps_set_eta(tn_eps* op, const void* eta)
     TN_API_END
 }
 
+int tn_eps_set_cell_eta(tn_eps* op, const void* eta_cell)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op, "null operator");
+    op->eta_cell = eta_cell;
+    op->blk_ready = false;
+    TN_API_END
+}
+
+int tn_eps_grad_log_cell_eta(tn_eps* op, const void* u, const void* lambda, void* grad, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op && u && lambda && grad, "null pointer");
+    TN_REQUIRE(op->eta_cell, "gradient: the operator has no per-cell viscosity (tn_eps_set_cell_eta)");
+    TN_REQUIRE(op->stored_dtype < 0, "gradient: the operator must be matrix-free");
+    const MeshLevel& m = const_cast<tn_mesh*>(op->mesh)->lvl.dev();
+    if (op->dtype == TN_F64)
+        tn::launch_grad_log_cell_eta<double>(geom_of<double>(m), (const double*)op->eta_cell, (const double*)u,
+                                             (const double*)lambda, op->bc_cmb, op->bc_surf, (double*)grad,
+                                             (hipStream_t)s);
+    else
+        tn::launch_grad_log_cell_eta<float>(geom_of<float>(m), (const float*)op->eta_cell, (const float*)u,
+                                            (const float*)lambda, op->bc_cmb, op->bc_surf, (float*)grad,
+                                            (hipStream_t)s);
+    TN_API_END
+}
+
 int tn_eps_apply(tn_eps* op, const void* src, void* dst, tn_stream_t s)
 {
     TN_API_BEGIN
@@ -422,12 +471,13 @@ int tn_eps_gca_from(tn_eps* coarse, const tn_eps* fine, int store_dtype, tn_stre
     coarse->stored.alloc((size_t)bytes);
     coarse->stored_dtype = store_dtype;
     const bool fine_geo = fine->stored_dtype < 0;
-    TN_REQUIRE(!fine_geo || fine->eta, "GCA: fine operator has no viscosity");
+    TN_REQUIRE(!fine_geo || fine->eta || fine->eta_cell, "GCA: fine operator has no viscosity");
     auto go = [&](auto* storec, auto* eta_f, auto* stored_f) {
         using S = std::remove_pointer_t<decltype(storec)>;
         using ET = std::remove_const_t<std::remove_pointer_t<decltype(eta_f)>>;
         using SF = std::remove_const_t<std::remove_pointer_t<decltype(stored_f)>>;
-        tn::GcaArgs<ET, S, SF> a{mc.n, mc.nr, mf.n, mf.nr, mf.d_geo.p, mf.d_radii.p, eta_f, stored_f, storec};
+        tn::GcaArgs<ET, S, SF> a{mc.n,  mc.nr,  mf.n, mf.nr, mf.d_geo.p, mf.d_radii.p, eta_f,
+                                 static_cast<const ET*>(fine->eta_cell), stored_f, storec};
         tn::launch_gca_build<ET, S, SF>(a, fine_geo, (hipStream_t)s);
     };
     const bool f64e = fine->dtype == TN_F64;

@@ -84,6 +84,7 @@ struct tn_gmg {
     std::vector<BVec> invd, ta, tb, tmp, r, e, ctmp;
     std::vector<std::unique_ptr<Chebyshev>> sm;
     std::unique_ptr<PCG> coarse;
+    std::unique_ptr<PrecisionBridge> coarse_bridge;  // coarsest level in another precision
     std::unique_ptr<Multigrid> mg;
     Ctx ctx;
 };
@@ -116,8 +117,10 @@ void gmg_setup(tn_gmg* g, const tn_gmg_options& o, hipStream_t s)
         g->tb.push_back(g->store.make(like));
         g->tmp.push_back(g->store.make(like));
         if (l < L - 1) {
-            g->r.push_back(g->store.make(like));
-            g->e.push_back(g->store.make(like));
+            // level vectors between transfers carry the hierarchy's precision
+            const BVec hl = vec1(field_of(op->mesh, g->dtype, 3, nullptr));
+            g->r.push_back(g->store.make(hl));
+            g->e.push_back(g->store.make(hl));
         }
         g->sm.emplace_back(new Chebyshev(o.cheby_order, o.cheby_iterations, g->invd.back(), g->ta.back(), g->tb.back(),
                                          o.power_iterations, o.seed + 1000003ull * l));
@@ -129,6 +132,13 @@ void gmg_setup(tn_gmg* g, const tn_gmg_options& o, hipStream_t s)
     kp.rtol = o.coarse_rtol;
     kp.atol = o.coarse_atol;
     g->coarse.reset(new PCG(kp, g->ctmp, nullptr));
+    Solver* coarse = g->coarse.get();
+    if (g->ops[0]->dtype != g->dtype) {
+        // e.g. FP32 V-cycle with an FP64 coarsest-grid solve
+        g->coarse_bridge.reset(new PrecisionBridge(g->coarse.get(), g->lops[0].get(), g->store.make(like0),
+                                                   g->store.make(like0)));
+        coarse = g->coarse_bridge.get();
+    }
     std::vector<LinOp*> lo;
     std::vector<Transfer> tr;
     std::vector<Solver*> sms;
@@ -138,7 +148,7 @@ void gmg_setup(tn_gmg* g, const tn_gmg_options& o, hipStream_t s)
         if (l < L - 1)
             tr.push_back(Transfer{&const_cast<tn_mesh*>(g->ops[l]->mesh)->lvl, &const_cast<tn_mesh*>(g->ops[l + 1]->mesh)->lvl});
     }
-    g->mg.reset(new Multigrid(lo, tr, sms, g->coarse.get(), g->tmp, g->r, g->e, o.num_cycles));
+    g->mg.reset(new Multigrid(lo, tr, sms, coarse, g->tmp, g->r, g->e, o.num_cycles));
 }
 
 void fill_info(tn_solve_info* info, const IterInfo& it, double secs, double* hist, int cap)
@@ -162,9 +172,11 @@ int tn_gmg_create(int nlevels, tn_eps* const* ops, const tn_gmg_options* opts, t
     TN_API_BEGIN
     TN_REQUIRE(ops && opts && out && nlevels >= 2, "need at least 2 levels");
     auto g = std::make_unique<tn_gmg>();
-    g->dtype = ops[0]->dtype;
+    TN_REQUIRE(ops[nlevels - 1], "null operator");
+    g->dtype = ops[nlevels - 1]->dtype;
     for (int l = 0; l < nlevels; ++l) {
-        TN_REQUIRE(ops[l] && ops[l]->dtype == g->dtype, "all levels must share one dtype");
+        // the coarsest level alone may use another precision (its solve is bridged)
+        TN_REQUIRE(ops[l] && (l == 0 || ops[l]->dtype == g->dtype), "levels above the coarsest must share one dtype");
         if (l > 0) {
             const auto& mc = ops[l - 1]->mesh->lvl;
             const auto& mf = ops[l]->mesh->lvl;

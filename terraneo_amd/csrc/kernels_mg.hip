@@ -238,9 +238,14 @@ __global__ void k_gca_build(GcaArgs<T, S, SF> a)
         if constexpr (FINE_GEO) {
             const double* geo_w = a.geo_f + ((((int64_t)d * ncf + fx) * ncf + fy) * 2 + w) * GEO_STRIDE;
             double ksum = 0.0;
-            for (int j = 0; j < 6; ++j)
-                ksum += (double)a.eta_f[(((int64_t)d * a.nf + fx + WOFF[w][j][0]) * a.nf + fy + WOFF[w][j][1]) * a.nrf +
-                                        fr + WOFF[w][j][2]];
+            if (a.eta_cell_f) {
+                ksum = 6.0 * (double)a.eta_cell_f[fcell];
+            } else {
+                for (int j = 0; j < 6; ++j)
+                    ksum += (double)a.eta_f[(((int64_t)d * a.nf + fx + WOFF[w][j][0]) * a.nf + fy + WOFF[w][j][1]) *
+                                                a.nrf +
+                                            fr + WOFF[w][j][2]];
+            }
             eps_block_geo<double>(geo_w, a.radii_f[fr], a.radii_f[fr + 1], ksum, q, p, A);
         } else {
             const int64_t ncellf = 10LL * ncf * ncf * ncrf;

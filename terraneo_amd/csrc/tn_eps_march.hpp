@@ -5,13 +5,14 @@
 
 namespace tn {
 
-constexpr int MARCH_WAVES = 8;  // waves per block; MARCH_WAVES-1 node rows owned per block
+constexpr int MARCH_SEG_MAX = 64; // node columns per block segment, at most
 
 template <typename T>
 struct MarchArgs {
     const T* src;             // [3][N]
     T* dst;                   // [3][N]
-    const T* eta;             // [N]
+    const T* eta;             // [N] nodal viscosity
+    const T* eta_cell;        // [10][n-1][n-1][nr-1] per-cell viscosity, or null (nodal)
     const T* geo;             // [10][n-1][n-1][2][GEO_STRIDE]
     const T* coords;          // [10][n][n][3] unit lateral coordinates
     const T* radii;           // [nr]
@@ -25,11 +26,13 @@ struct MarchArgs {
     int add;                  // 1: dst += A x, 0: dst = A x
 };
 
-bool march_supported(int nr);
+// node rows owned per block (waves per block - 1) for an element size in bytes
+int march_block_rows(int dtype_size);
 
 // boundary-node quadratic forms e^T B e (see k_boundary_blocks)
 template <typename T>
-void launch_boundary_blocks(const Geom<T>& g, const T* eta, int bc_cmb, int bc_surf, T* blk, hipStream_t s);
+void launch_boundary_blocks(const Geom<T>& g, const T* eta, const T* eta_cell, int bc_cmb, int bc_surf, T* blk,
+                            hipStream_t s);
 
 template <typename T>
 void launch_eps_march(const MarchArgs<T>& a, bool diagonal, hipStream_t s);

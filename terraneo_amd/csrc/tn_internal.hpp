@@ -113,13 +113,10 @@ struct Geom {
     int n, nr;
 };
 
-struct EpsLaunch {
-    int bc_cmb = TN_NEUMANN, bc_surf = TN_NEUMANN;
-    bool diagonal = false;
-};
-
+// adjoint gradient w.r.t. per-cell log-viscosity: grad[cell] = -eta_c (T lam)^T A_c(1) u
 template <typename T>
-void launch_eps_apply(const Geom<T>& g, const T* eta, const T* src, T* dst, const EpsLaunch& cfg, hipStream_t s);
+void launch_grad_log_cell_eta(const Geom<T>& g, const T* eta_cell, const T* u, const T* lam, int bc_cmb, int bc_surf,
+                              T* grad, hipStream_t s);
 
 template <typename T>
 void launch_fill(T* y, int64_t count, T value, hipStream_t s);

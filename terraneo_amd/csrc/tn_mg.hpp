@@ -52,6 +52,7 @@ struct GcaArgs {
     const double* geo_f;    // fine geometry table (fine level matrix-free)
     const double* radii_f;
     const T* eta_f;         // fine nodal viscosity
+    const T* eta_cell_f;    // fine per-cell viscosity (overrides eta_f when set)
     const SF* stored_f; // fine packed storage (fine level stored)
     S* stored_c;        // coarse packed storage (output)
 };

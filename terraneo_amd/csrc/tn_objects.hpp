@@ -6,7 +6,8 @@
 struct tn_eps {
     const tn_mesh* mesh = nullptr;
     int dtype = TN_F64;
-    const void* eta = nullptr;
+    const void* eta = nullptr;       // nodal viscosity [10][n][n][nr]
+    const void* eta_cell = nullptr;  // per hex cell viscosity [10][n-1][n-1][nr-1]; overrides eta when set
     int bc_cmb = TN_NEUMANN, bc_surf = TN_NEUMANN;
     int apply_mode = TN_REPLACE, comm_mode = TN_COMM_ADDITIVE;
     bool diagonal = false;

@@ -45,6 +45,9 @@ def lib():
         L.orc_eps_destroy.argtypes = [vp]
         L.orc_eps_apply.argtypes = [vp, dp, dp, C.c_int, C.c_int]
         L.orc_eps_set_stored.argtypes = [vp, dp]
+        L.orc_eps_local_matrix_k.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp]
+        L.orc_eps_set_cell_viscosity.argtypes = [vp, dp]
+        L.orc_eps_grad_log_cell_viscosity.argtypes = [vp, dp, dp, dp]
         L.orc_eps_penalty_modes.argtypes = [vp, dp]
         L.orc_eps_penalty_modes.restype = C.c_int
         L.orc_div_apply.argtypes = [vp, vp, C.c_int, C.c_int, dp, dp, C.c_int]
@@ -123,10 +126,25 @@ class Mesh:
 class Eps:
     """Oracle epsilon-div-div operator (assembled local-matrix formulation)."""
 
-    def __init__(self, mesh, k, bc_cmb=NEUMANN, bc_surf=NEUMANN):
+    def __init__(self, mesh, k, bc_cmb=NEUMANN, bc_surf=NEUMANN, kc=None):
+        """k: nodal viscosity; kc (optional): per hex cell viscosity, overriding k."""
         self.m = mesh
+        if k is None:
+            k = np.ones(10 * mesh.n * mesh.n * mesh.nr)
         self.k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1)
         self.h = lib().orc_eps_create(mesh.h, self.k, bc_cmb, bc_surf)
+        self.kc = None
+        if kc is not None:
+            self.kc = np.ascontiguousarray(kc, dtype=np.float64).reshape(-1)
+            lib().orc_eps_set_cell_viscosity(self.h, self.kc)
+
+    def grad_log_cell_viscosity(self, u, lam):
+        """g[cell] = -kc[cell] sum_wedges (T lam)_e^T A_e(1) u_e."""
+        assert self.kc is not None
+        g = np.zeros(10 * (self.m.n - 1) ** 2 * (self.m.nr - 1))
+        lib().orc_eps_grad_log_cell_viscosity(self.h, np.ascontiguousarray(u, dtype=np.float64).reshape(-1),
+                                              np.ascontiguousarray(lam, dtype=np.float64).reshape(-1), g)
+        return g
 
     def apply(self, x, diagonal=False, comm=True):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)

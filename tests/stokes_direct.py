@@ -43,3 +43,20 @@ def solve_assess_dirichlet(level):
     ev = np.sqrt(np.sum((u - ue) ** 2) / nu)
     ep = np.sqrt(np.sum((p - pe) ** 2) / npr)
     return ev, ep, dict(mv=mv, mp=mp, u=u, p=p, K=K, rhs=rhs, sol=sol)
+
+
+def stokes_matrix(mv, mp, eps_op, bc):
+    """[A B^T 0; B 0 c; 0 c^T 0] over unique dofs (c: mean-zero pressure row),
+    A from the oracle operator `eps_op` (its boundary conditions), B with `bc`."""
+    A = O.eps_assemble(eps_op)
+    B = O.div_assemble(mv, mp, *bc)
+    npr = B.shape[0]
+    c = sp.csr_matrix(np.ones((1, npr)))
+    return sp.bmat([[A, B.T, None], [B, None, c.T], [None, c, None]], format="csc"), A.shape[0], npr
+
+
+def solve_unique(K, nu, npr, mv, mp, bu, bp):
+    """Solve K (u, p, mu) = (bu, bp, 0); inputs/outputs in the duplicated layout."""
+    rhs = np.concatenate([O.to_unique(mv, bu, 3), O.to_unique(mp, bp, 1), [0.0]])
+    sol = spl.spsolve(K, rhs)
+    return O.from_unique(mv, sol[:nu], 3), O.from_unique(mp, sol[nu:nu + npr], 1)

@@ -15,23 +15,29 @@ BCS = [(0, 0), (1, 1), (2, 2), (2, 1), (1, 2), (0, 1)]
 
 
 def _setup(lat, rad, seed=0):
+    """rad: radial refinement level (uniform radii) or an explicit radius list."""
     import terraneo_amd as tn
 
-    m = tn.Mesh(lat, rad, 0.55, 1.0)
-    om = O.Mesh(lat, m.nr, 0.55, 1.0)
+    if isinstance(rad, (list, tuple)):
+        m = tn.Mesh(lat, radii=np.array(rad))
+        om = O.Mesh(lat, radii=np.array(rad))
+    else:
+        m = tn.Mesh(lat, rad, 0.55, 1.0)
+        om = O.Mesh(lat, m.nr, 0.55, 1.0)
     eta = smooth_eta(om.coords, om.radii).reshape(-1)
     x = consistent_random(om, 3, seed).reshape(-1)
     return m, om, eta, x
 
 
-# shapes: 1-2 radial passes of the lateral-march kernel (64 / 128 radial cells),
-# several lateral blocks and march segments, and the single-layer shell (the
-# atomic kernel's case)
-SHAPES = [(2, 2), (3, 3), (4, 4), (2, 7), (3, 6), (5, 2), (1, 1), (2, 0)]
+# shapes: 1-4 radial chunks of the lateral-march kernel (64 radial cells per
+# chunk, 64 and 128 exactly), several lateral blocks and march segments, the
+# smallest mesh (2 radial cells) and graded radii
+GRADED = list(0.55 + 0.45 * (np.linspace(0.0, 1.0, 12) ** 1.7))
+SHAPES = [(2, 2), (3, 3), (4, 4), (2, 7), (3, 6), (5, 2), (1, 1), (2, 8), (3, GRADED)]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lat,rad", SHAPES)
+@pytest.mark.parametrize("lat,rad", SHAPES, ids=lambda v: "graded" if isinstance(v, list) else str(v))
 @pytest.mark.parametrize("bc", BCS)
 @pytest.mark.parametrize("diag", [False, True])
 def test_eps_apply_fp64_matches_oracle(cuda, lat, rad, bc, diag):
