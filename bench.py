@@ -35,7 +35,69 @@ def parse():
     p.add_argument("--rad", type=int, default=7)
     p.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-solve", action="store_true", help="skip the FP64 / mixed-precision Stokes solves")
+    p.add_argument("--solve-lat", type=int, default=7)
+    p.add_argument("--solve-rad", type=int, default=7)
+    p.add_argument("--solve-levels", type=int, default=5)
     return p.parse_args()
+
+
+def solve_bench(args):
+    """Free-slip Stokes solve to relative residual 1e-8: FP64 multigrid vs the
+    mixed-precision one (FP32 V-cycle, FP32 Galerkin matrices, FP64 coarsest
+    solve) inside FP64 FGMRES(30), velocity lateral level 7 x 129 radial nodes,
+    viscosity contrast 1e5 (random harmonics, seed 3) x30 for r > 0.9, buoyancy
+    forcing.  Setup (meshes, GCA, Chebyshev eigenvalue estimates) is excluded;
+    the timed region is the Krylov solve from a zero initial guess."""
+    import gc
+
+    import numpy as np
+    import torch
+
+    from terraneo_amd import api
+    from terraneo_amd import workloads as W
+    from terraneo_amd.problem import StokesProblem
+
+    dev = torch.device("cuda:0")
+    out = {}
+    sols = {}
+    for prec in ("f64", "f32"):
+        pr = StokesProblem(args.solve_lat, args.solve_rad, args.solve_levels, device=dev, prec_dtype=prec,
+                           gmg=api.gmg_options(cheby_order=3, cheby_iterations=1, power_iterations=10))
+        coords, radii = pr.coords()
+        eta = torch.from_numpy(W.viscosity_c2(coords, radii).reshape(-1)).to(dev)
+        f = torch.from_numpy(W.buoyancy(coords, radii)).to(dev)
+        t0 = time.perf_counter()
+        pr.set_viscosity(eta=eta)
+        bu, bp = pr.rhs_from_force(f)
+        pr.solve(bu, bp, rtol=1e-8, max_iterations=1)  # eigenvalue estimates (setup)
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        u, p, info = pr.solve(bu, bp, rtol=1e-8, max_iterations=1000, restart=30)
+        res = pr.residual(u, p, bu, bp)
+        out[prec] = {"seconds": round(info["seconds"], 4), "iterations": info["iterations"],
+                     "true_rel_residual": float(f"{res:.3e}"), "setup_seconds": round(setup, 3)}
+        if prec == "f64":
+            sols["u"] = u.clone()
+            sols["pr"] = pr
+        else:
+            du = u - sols["u"]
+            m = sols["pr"].vel
+            out["velocity_rel_l2_vs_fp64"] = float(f"{np.sqrt(m.dot(du, du, 3) / m.dot(sols['u'], sols['u'], 3)):.3e}")
+        del pr, u, p, bu, bp, eta, f
+        gc.collect()
+        torch.cuda.empty_cache()
+    n = 2 ** args.solve_lat + 1
+    nr = 2 ** args.solve_rad + 1
+    return {
+        "workload": f"free-slip Stokes, velocity lateral level {args.solve_lat} ({n}x{n} per diamond) x {nr} radial, "
+                    f"{args.solve_levels}-level GMG (Chebyshev degree 3), FGMRES(30) block-triangular, rel. res 1e-8, "
+                    "viscosity contrast 1e5 x30 (r > 0.9)",
+        "fp64_gmg": out["f64"],
+        "mixed_gmg": out["f32"],
+        "mixed_over_fp64_time": round(out["f32"]["seconds"] / out["f64"]["seconds"], 4),
+        "velocity_rel_l2_vs_fp64": out["velocity_rel_l2_vs_fp64"],
+    }
 
 
 def cpu_baseline(seconds=10.0):
@@ -146,6 +208,12 @@ def main():
         except Exception:
             traffic = None
 
+    solve = None
+    if not args.no_solve:
+        del x, y, eta, op
+        torch.cuda.empty_cache()
+        solve = solve_bench(args)
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -183,6 +251,7 @@ def main():
             "traffic": traffic,
         },
         "cpu_baseline": cpu,
+        "solve": solve,
     }
     print(json.dumps(out))
 

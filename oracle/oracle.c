@@ -1160,7 +1160,16 @@ void orc_restrict(const orc_mesh* mf, const orc_mesh* mc, int ncomp, const doubl
  * bracket r_bot = r_fine/2 (gca.hpp:213-216) that, for in-plane midpoints on
  * the top plane of an interior coarse cell, places the weights in the bottom
  * coarse dofs (recorded as finding F11). */
-void orc_gca_build(const orc_eps* fine, const orc_mesh* mc, double* out)
+static void gca_build(const orc_eps* fine, const orc_mesh* mc, double* out, int geometric_bracket);
+
+void orc_gca_build(const orc_eps* fine, const orc_mesh* mc, double* out) { gca_build(fine, mc, out, 0); }
+
+/* same with the radial bracket of every fine dof taken as the coarse cell it
+ * lies in (the Galerkin product of the global constant prolongation); test
+ * infrastructure for the brute-force check of the element loop */
+void orc_gca_build_geometric(const orc_eps* fine, const orc_mesh* mc, double* out) { gca_build(fine, mc, out, 1); }
+
+static void gca_build(const orc_eps* fine, const orc_mesh* mc, double* out, int geometric_bracket)
 {
     const orc_mesh* mf = fine->m;
     static const int SH[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
@@ -1178,7 +1187,8 @@ void orc_gca_build(const orc_eps* fine, const orc_mesh* mc, double* out)
                             for (int fl = 0; fl < 6; fl++) {
                                 const int ix = fx + WOFF[w][fl][0], iy = fy + WOFF[w][fl][1], ir = fr + WOFF[w][fl][2];
                                 if (ix % 2 == 0 && iy % 2 == 0 && ir % 2 == 0) { P[fl * 6 + fl] = 1.0; continue; }
-                                const int rb = ir < mf->nr - 1 ? ir / 2 : ir / 2 - 1, rt = rb + 1;
+                                const int rb = geometric_bracket ? Rr : (ir < mf->nr - 1 ? ir / 2 : ir / 2 - 1);
+                                const int rt = rb + 1;
                                 if (ix % 2 == 0 && iy % 2 == 0) {
                                     const int a = fl % 3;
                                     P[fl * 6 + a] = 0.5;

@@ -105,6 +105,8 @@ void orc_prolongate_add(const orc_mesh* mc, const orc_mesh* mf, int ncomp, const
 void orc_restrict(const orc_mesh* mf, const orc_mesh* mc, int ncomp, const double* xf, double* xc);
 /* coarse stored matrices from fine op: out [10][nc-1][nc-1][nrc-1][2][324] */
 void orc_gca_build(const orc_eps* fine, const orc_mesh* mc, double* out);
+/* variant with the geometric radial bracket (the exact Galerkin product P^T A P) */
+void orc_gca_build_geometric(const orc_eps* fine, const orc_mesh* mc, double* out);
 
 /* ---------------------------------------------------------------- BLAS-1 */
 double orc_dot_owned(const orc_mesh* m, int ncomp, const double* a, const double* b);

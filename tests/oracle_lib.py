@@ -57,6 +57,7 @@ def lib():
         L.orc_prolongate_add.argtypes = [vp, vp, C.c_int, dp, dp]
         L.orc_restrict.argtypes = [vp, vp, C.c_int, dp, dp]
         L.orc_gca_build.argtypes = [vp, vp, dp]
+        L.orc_gca_build_geometric.argtypes = [vp, vp, dp]
         L.orc_dot_owned.argtypes = [vp, C.c_int, dp, dp]
         L.orc_dot_owned.restype = C.c_double
         L.orc_interface_sum.argtypes = [C.c_int, C.c_int, C.c_int, ip, C.c_int, dp]
@@ -209,10 +210,11 @@ def restrict(mf, mc, xf, ncomp=3):
     return xc
 
 
-def gca_build(fine_eps, mc):
+def gca_build(fine_eps, mc, geometric=False):
+    """Stored coarse matrices; geometric=True: exact Galerkin variant (test only)."""
     cells = 10 * (mc.n - 1) ** 2 * (mc.nr - 1)
     out = np.zeros(cells * 2 * 324)
-    lib().orc_gca_build(fine_eps.h, mc.h, out)
+    (lib().orc_gca_build_geometric if geometric else lib().orc_gca_build)(fine_eps.h, mc.h, out)
     return out
 
 

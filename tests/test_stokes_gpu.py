@@ -68,40 +68,58 @@ def test_mass_operators_match_oracle(cuda):
     assert np.abs(d.cpu().numpy() - refd).max() <= 1e-13 * np.abs(refd).max()
 
 
-def assess_solve_gpu(cuda, level, min_level=2):
-    """Reference assess test on the GPU: EpsDivDivStokes, FGMRES(30) rtol 1e-14,
+def assess_solve_gpu(cuda, level, min_level=2, kind="zeroslip"):
+    """Reference assess tests on the GPU: EpsDivDivStokes, FGMRES(30) rtol 1e-14,
     block-triangular preconditioner, V-cycle x2 with Chebyshev(2) x3, constant
     transfers, re-discretised coarse operators, PCG coarsest (1000, 1e-6),
-    Schur = lumped mass (eta = 1)."""
+    Schur = lumped mass (eta = 1).
+      zeroslip      Dirichlet both shells, lifting of the exact boundary values
+                    (test_epsilon_divdiv_stokes_assess.cpp)
+      freeslip      free-slip both shells with the rotation penalty, RHS zeroed
+                    on both shells (test_epsilon_divdiv_stokes_assess_freeslip_freeslip.cpp:592-606)
+      freezeroslip  free-slip CMB (RHS normal part removed), zero-slip surface
+                    (RHS zeroed) (test_epsilon_divdiv_stokes_assess_freeslip.cpp:592-610)"""
     import torch
 
     import terraneo_amd as tn
 
+    sol = AS.Solution(kind)
+    bc = {"zeroslip": (tn.TN_DIRICHLET, tn.TN_DIRICHLET), "freeslip": (tn.TN_FREESLIP, tn.TN_FREESLIP),
+          "freezeroslip": (tn.TN_FREESLIP, tn.TN_DIRICHLET)}[kind]
     rmin, rmax = AS.R_MIN, AS.R_MAX
     meshes = [tn.Mesh(l, l, rmin, rmax) for l in range(min_level, level + 1)]
     mv, mp = meshes[-1], tn.Mesh(level - 1, level - 1, rmin, rmax)
     ov, op = O.Mesh(level, mv.nr, rmin, rmax), O.Mesh(level - 1, mp.nr, rmin, rmax)
     f64 = dict(dtype=torch.float64, device=cuda)
-    uex = torch.from_numpy(np.ascontiguousarray(AS.velocity(ov.node_xyz()).reshape(-1))).to(cuda)
-    pex = torch.from_numpy(np.ascontiguousarray(AS.pressure(op.node_xyz()).reshape(-1))).to(cuda)
-    fn = torch.from_numpy(np.ascontiguousarray(AS.body_force(ov.node_xyz()).reshape(-1))).to(cuda)
+    uex = torch.from_numpy(np.ascontiguousarray(sol.velocity(ov.node_xyz()).reshape(-1))).to(cuda)
+    pex = torch.from_numpy(np.ascontiguousarray(sol.pressure(op.node_xyz()).reshape(-1))).to(cuda)
+    fn = torch.from_numpy(np.ascontiguousarray(sol.body_force(ov.node_xyz()).reshape(-1))).to(cuda)
     ones = [torch.ones(m.N, **f64) for m in meshes]
     f = torch.empty(3 * mv.N, **f64)
     tn.api.vector_mass(mv, fn, f)
-    g = torch.zeros(3 * mv.N, **f64)
-    tn.api.boundary_assign(mv, g, uex, 3)
-    AN = tn.Eps(mv, ones[-1], tn.TN_NEUMANN, tn.TN_NEUMANN)
-    KN = tn.Stokes(mv, mp, AN)
     zp = torch.zeros(mp.N, **f64)
-    ku, kp = torch.empty_like(f), torch.empty_like(zp)
-    KN.apply(g, zp, ku, kp)
-    bu = f - ku
-    bp = -kp
-    AN.set_diagonal(True)
-    dg = torch.empty_like(f)
-    AN.apply(g, dg)
-    tn.api.boundary_assign(mv, bu, dg, 3)
-    ops = [tn.Eps(m, e, tn.TN_DIRICHLET, tn.TN_DIRICHLET) for m, e in zip(meshes, ones)]
+    if kind == "zeroslip":
+        g = torch.zeros(3 * mv.N, **f64)
+        tn.api.boundary_assign(mv, g, uex, 3)
+        AN = tn.Eps(mv, ones[-1], tn.TN_NEUMANN, tn.TN_NEUMANN)
+        KN = tn.Stokes(mv, mp, AN)
+        ku, kp = torch.empty_like(f), torch.empty_like(zp)
+        KN.apply(g, zp, ku, kp)
+        bu = f - ku
+        bp = -kp
+        AN.set_diagonal(True)
+        dg = torch.empty_like(f)
+        AN.apply(g, dg)
+        tn.api.boundary_assign(mv, bu, dg, 3)
+    else:
+        bu, bp = f, zp.clone()
+        zero = torch.zeros_like(f)
+        if kind == "freeslip":
+            tn.api.boundary_assign(mv, bu, zero, 3)
+        else:
+            tn.api.freeslip_project(mv, bu, 1)
+            tn.api.boundary_assign(mv, bu, zero, 2)
+    ops = [tn.Eps(m, e, *bc) for m, e in zip(meshes, ones)]
     mg = tn.Gmg(ops, tn.api.gmg_options(cheby_order=2, cheby_iterations=3, power_iterations=10, num_cycles=2))
     K = tn.Stokes(mv, mp, ops[-1])
     mass = torch.empty(mp.N, **f64)
@@ -125,6 +143,19 @@ def test_assess_stokes_solve_matches_reference_errors(cuda, level):
     ev, ep, info = assess_solve_gpu(cuda, level)
     rv, rp = AS.REFERENCE_ERRORS_DIRICHLET[level]
     print(f"level {level}: vel {ev:.6e} (ref {rv}) pre {ep:.6e} (ref {rp}) its {info['iterations']} "
+          f"res {info['final_residual'] / info['initial_residual']:.2e} t {info['seconds']:.2f}s")
+    assert info["final_residual"] / info["initial_residual"] < 1e-12
+    assert abs(ev - rv) / rv < 1e-4, (ev, rv)
+    assert abs(ep - rp) / rp < 1e-4, (ep, rp)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,level", [("freeslip", 3), ("freeslip", 4), ("freezeroslip", 3), ("freezeroslip", 4)])
+def test_assess_freeslip_solve_matches_reference_errors(cuda, kind, level):
+    ev, ep, info = assess_solve_gpu(cuda, level, kind=kind)
+    ref = AS.REFERENCE_ERRORS_FREESLIP if kind == "freeslip" else AS.REFERENCE_ERRORS_FREEZEROSLIP
+    rv, rp = ref[level]
+    print(f"{kind} level {level}: vel {ev:.6e} (ref {rv}) pre {ep:.6e} (ref {rp}) its {info['iterations']} "
           f"res {info['final_residual'] / info['initial_residual']:.2e} t {info['seconds']:.2f}s")
     assert info["final_residual"] / info["initial_residual"] < 1e-12
     assert abs(ev - rv) / rv < 1e-4, (ev, rv)
