@@ -39,7 +39,13 @@ def parse():
     p.add_argument("--solve-lat", type=int, default=7)
     p.add_argument("--solve-rad", type=int, default=7)
     p.add_argument("--solve-levels", type=int, default=5)
+    p.add_argument("--solve-max-iterations", type=int, default=300)
     return p.parse_args()
+
+
+def log(msg):
+    """progress line on stderr (the JSON result is the only stdout line)"""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def solve_bench(args):
@@ -73,7 +79,9 @@ def solve_bench(args):
         pr.solve(bu, bp, rtol=1e-8, max_iterations=1)  # eigenvalue estimates (setup)
         torch.cuda.synchronize()
         setup = time.perf_counter() - t0
-        u, p, info = pr.solve(bu, bp, rtol=1e-8, max_iterations=1000, restart=30)
+        log(f"solve {prec}: setup {setup:.2f} s")
+        u, p, info = pr.solve(bu, bp, rtol=1e-8, max_iterations=args.solve_max_iterations, restart=30)
+        log(f"solve {prec}: {info['iterations']} iterations, {info['seconds']:.3f} s")
         res = pr.residual(u, p, bu, bp)
         out[prec] = {"seconds": round(info["seconds"], 4), "iterations": info["iterations"],
                      "true_rel_residual": float(f"{res:.3e}"), "setup_seconds": round(setup, 3)}
@@ -162,6 +170,7 @@ def main():
     op = tn.Eps(mesh, eta, tn.TN_DIRICHLET, tn.TN_DIRICHLET)
 
     stream = torch.cuda.current_stream()
+    log(f"apply: lateral {args.lat} x {mesh.nr} radial, {args.dtype}")
     for _ in range(args.warmup):
         op.apply(x, y)
     torch.cuda.synchronize()
@@ -208,6 +217,7 @@ def main():
         except Exception:
             traffic = None
 
+    log(f"apply: {ms_per_step:.4f} ms per apply, kernel {kernel_ms:.4f} ms")
     solve = None
     if not args.no_solve:
         del x, y, eta, op
@@ -216,6 +226,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, bounded sample)")
         try:
             cpu = cpu_baseline()
         except Exception as e:  # the baseline never blocks the GPU number

@@ -58,7 +58,17 @@ typedef struct tn_gmg_options {
     double coarse_rtol;
     double coarse_atol;
     uint64_t seed;             /* power-iteration start vector               */
+    int coarse_solver;         /* TN_COARSE_PCG (the reference's) or TN_COARSE_DENSE */
 } tn_gmg_options;
+
+/* coarsest-level solver of the multigrid: unpreconditioned PCG as the
+ * reference, or a dense LU inverse of the assembled coarsest operator
+ * (rocSOLVER; at most 8000 owned coarse nodes) */
+enum tn_coarse_solver { TN_COARSE_PCG = 0, TN_COARSE_DENSE = 1 };
+/* radial bracket of the Galerkin coarsening's local prolongation:
+ * TN_GCA_REFERENCE reproduces gca.hpp:213-216 (r_fine/2), TN_GCA_GEOMETRIC
+ * uses the coarse cell the fine node lies in (exact P^T A P) */
+enum tn_gca_bracket { TN_GCA_REFERENCE = 0, TN_GCA_GEOMETRIC = 1 };
 
 /* IterativeSolverParameters / FGMRESOptions (iterative_solver_info.hpp, fgmres.hpp:20-44) */
 typedef struct tn_krylov_options {
@@ -152,6 +162,8 @@ int tn_eps_set_penalty_epsilon(tn_eps* op, double eps);
  * switches `coarse` to its stored-matrix path, set_stored_matrix_mode kerngen.hpp:443-464).
  * Matrices are stored symmetric-packed in TN_F64 or TN_F32. */
 int tn_eps_gca_from(tn_eps* coarse, const tn_eps* fine, int store_dtype, tn_stream_t s);
+/* same with an explicit radial bracket (tn_gca_bracket); tn_eps_gca_from == TN_GCA_REFERENCE */
+int tn_eps_gca_from_bracket(tn_eps* coarse, const tn_eps* fine, int store_dtype, int bracket, tn_stream_t s);
 /* get_local_matrix of a stored operator (kerngen.hpp:481-495), host out[18*18],
  * dimension-wise dof order (component*6 + wedge node); synchronous */
 int tn_eps_get_local_matrix(const tn_eps* op, int sd, int x, int y, int r, int wedge, double* out);

@@ -13,12 +13,14 @@ TN_NEUMANN, TN_DIRICHLET, TN_FREESLIP = 0, 1, 2
 TN_REPLACE, TN_ADD = 0, 1
 TN_COMM_SKIP, TN_COMM_ADDITIVE = 0, 1
 TN_MINRES_BLOCKDIAG, TN_FGMRES_BLOCKTRI, TN_FGMRES_BLOCKDIAG = 0, 1, 2
+TN_COARSE_PCG, TN_COARSE_DENSE = 0, 1
+TN_GCA_REFERENCE, TN_GCA_GEOMETRIC = 0, 1
 
 
 class GmgOptions(C.Structure):
     _fields_ = [("cheby_order", C.c_int), ("cheby_iterations", C.c_int), ("power_iterations", C.c_int),
                 ("num_cycles", C.c_int), ("coarse_max_iterations", C.c_int), ("coarse_rtol", C.c_double),
-                ("coarse_atol", C.c_double), ("seed", C.c_uint64)]
+                ("coarse_atol", C.c_double), ("seed", C.c_uint64), ("coarse_solver", C.c_int)]
 
 
 class KrylovOptions(C.Structure):
@@ -73,6 +75,7 @@ SIGNATURES = {
     "tn_eps_gca_from": (i32, [vp, vp, i32, vp]),
     "tn_eps_get_local_matrix": (i32, [vp, i32, i32, i32, i32, i32, vp]),
     "tn_eps_set_cell_eta": (i32, [vp, vp]),
+    "tn_eps_gca_from_bracket": (i32, [vp, vp, i32, i32, vp]),
     "tn_eps_grad_log_cell_eta": (i32, [vp, vp, vp, vp, vp]),
     "tn_stokes_create": (i32, [vp, vp, vp, pvp]),
     "tn_stokes_destroy": (i32, [vp]),

@@ -156,10 +156,13 @@ class Eps:
             pass
 
     # ---- Galerkin coarse operators
-    def gca_from(self, fine, store_dtype=TN_F64, stream=None):
-        """Make this operator's stored matrices the Galerkin coarsening of `fine`."""
+    def gca_from(self, fine, store_dtype=TN_F64, stream=None, bracket="reference"):
+        """Make this operator's stored matrices the Galerkin coarsening of `fine`
+        (bracket "reference": the reference's local prolongation, "geometric":
+        the exact Galerkin product)."""
         self.fine_ref = fine
-        check(lib().tn_eps_gca_from(self.h, fine.h, store_dtype, _stream_ptr(stream)))
+        b = {"reference": 0, "geometric": 1}[bracket]
+        check(lib().tn_eps_gca_from_bracket(self.h, fine.h, store_dtype, b, _stream_ptr(stream)))
 
     def local_matrix(self, sd, x, y, r, w):
         out = np.zeros(324)
@@ -231,9 +234,10 @@ class Stokes:
 
 
 def gmg_options(cheby_order=2, cheby_iterations=3, power_iterations=10, num_cycles=1, coarse_max_iterations=1000,
-                coarse_rtol=1e-6, coarse_atol=1e-16, seed=12345):
+                coarse_rtol=1e-6, coarse_atol=1e-16, seed=12345, coarse_solver="pcg"):
+    cs = {"pcg": 0, "dense": 1}[coarse_solver]
     return GmgOptions(cheby_order, cheby_iterations, power_iterations, num_cycles, coarse_max_iterations, coarse_rtol,
-                      coarse_atol, seed)
+                      coarse_atol, seed, cs)
 
 
 class Gmg:

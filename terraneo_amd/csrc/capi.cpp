@@ -109,7 +109,8 @@ int march_seg(int n, int dtype_size)
     }();
     if (env > 0) return std::min(std::min(env, n), tn::MARCH_SEG_MAX);
     const int own = tn::march_block_rows(dtype_size);
-    const int64_t nxb = (n + own - 1) / own, cus = device_cus();
+    const int64_t per_cu = std::max(1, tn::march_waves_per_cu(dtype_size) / (own + 1));
+    const int64_t nxb = (n + own - 1) / own, cus = device_cus() * per_cu;
     int best = std::min(n, tn::MARCH_SEG_MAX);
     int64_t best_cost = -1;
     for (int seg = std::min(n, tn::MARCH_SEG_MAX); seg >= std::min(n, 6); --seg) {
@@ -460,7 +461,13 @@ int tn_eps_set_penalty_epsilon(tn_eps* op, double eps)
 
 int tn_eps_gca_from(tn_eps* coarse, const tn_eps* fine, int store_dtype, tn_stream_t s)
 {
+    return tn_eps_gca_from_bracket(coarse, fine, store_dtype, TN_GCA_REFERENCE, s);
+}
+
+int tn_eps_gca_from_bracket(tn_eps* coarse, const tn_eps* fine, int store_dtype, int bracket, tn_stream_t s)
+{
     TN_API_BEGIN
+    TN_REQUIRE(bracket == TN_GCA_REFERENCE || bracket == TN_GCA_GEOMETRIC, "invalid GCA bracket");
     TN_REQUIRE(coarse && fine, "null operator");
     TN_REQUIRE(store_dtype == TN_F64 || store_dtype == TN_F32, "store dtype must be TN_F64 or TN_F32");
     auto& mc = const_cast<tn_mesh*>(coarse->mesh)->lvl.dev();
@@ -477,7 +484,7 @@ int tn_eps_gca_from(tn_eps* coarse, const tn_eps* fine, int store_dtype, tn_stre
         using ET = std::remove_const_t<std::remove_pointer_t<decltype(eta_f)>>;
         using SF = std::remove_const_t<std::remove_pointer_t<decltype(stored_f)>>;
         tn::GcaArgs<ET, S, SF> a{mc.n,  mc.nr,  mf.n, mf.nr, mf.d_geo.p, mf.d_radii.p, eta_f,
-                                 static_cast<const ET*>(fine->eta_cell), stored_f, storec};
+                                 static_cast<const ET*>(fine->eta_cell), stored_f, storec, bracket};
         tn::launch_gca_build<ET, S, SF>(a, fine_geo, (hipStream_t)s);
     };
     const bool f64e = fine->dtype == TN_F64;

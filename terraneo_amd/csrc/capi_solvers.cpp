@@ -3,6 +3,7 @@
 #include <chrono>
 #include <memory>
 
+#include "tn_coarse.hpp"
 #include "tn_objects.hpp"
 #include "tn_solvers.hpp"
 
@@ -85,6 +86,7 @@ struct tn_gmg {
     std::vector<std::unique_ptr<Chebyshev>> sm;
     std::unique_ptr<PCG> coarse;
     std::unique_ptr<PrecisionBridge> coarse_bridge;  // coarsest level in another precision
+    std::unique_ptr<DenseCoarse> dense;              // TN_COARSE_DENSE
     std::unique_ptr<Multigrid> mg;
     Ctx ctx;
 };
@@ -133,7 +135,11 @@ void gmg_setup(tn_gmg* g, const tn_gmg_options& o, hipStream_t s)
     kp.atol = o.coarse_atol;
     g->coarse.reset(new PCG(kp, g->ctmp, nullptr));
     Solver* coarse = g->coarse.get();
-    if (g->ops[0]->dtype != g->dtype) {
+    if (o.coarse_solver == TN_COARSE_DENSE) {
+        // gathers / scatters convert any precision: no bridge needed
+        g->dense.reset(new DenseCoarse(g->ops[0], s));
+        coarse = g->dense.get();
+    } else if (g->ops[0]->dtype != g->dtype) {
         // e.g. FP32 V-cycle with an FP64 coarsest-grid solve
         g->coarse_bridge.reset(new PrecisionBridge(g->coarse.get(), g->lops[0].get(), g->store.make(like0),
                                                    g->store.make(like0)));
@@ -171,6 +177,8 @@ int tn_gmg_create(int nlevels, tn_eps* const* ops, const tn_gmg_options* opts, t
 {
     TN_API_BEGIN
     TN_REQUIRE(ops && opts && out && nlevels >= 2, "need at least 2 levels");
+    TN_REQUIRE(opts->coarse_solver == TN_COARSE_PCG || opts->coarse_solver == TN_COARSE_DENSE,
+               "invalid coarse solver");
     auto g = std::make_unique<tn_gmg>();
     TN_REQUIRE(ops[nlevels - 1], "null operator");
     g->dtype = ops[nlevels - 1]->dtype;

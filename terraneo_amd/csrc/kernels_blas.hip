@@ -366,3 +366,62 @@ TN_INST(float)
 #undef TN_INST
 
 }  // namespace tn
+
+// ---------------------------------------------------------------- owned-copy gather / scatter (dense coarse solve)
+namespace tn {
+namespace {
+template <typename T>
+__global__ void k_gather_owned(const T* __restrict__ v, int64_t N, const int64_t* __restrict__ idx, int64_t nown,
+                               double* __restrict__ u)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * nown) return;
+    const int64_t k = i / 3;
+    const int c = (int)(i % 3);
+    u[i] = (double)v[c * N + idx[k]];
+}
+template <typename T>
+__global__ void k_scatter_owned(T* __restrict__ v, int64_t N, const int64_t* __restrict__ idx, int64_t nown,
+                                const double* __restrict__ u)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * nown) return;
+    const int64_t k = i / 3;
+    const int c = (int)(i % 3);
+    v[c * N + idx[k]] = (T)u[i];
+}
+template <typename T>
+__global__ void k_set_owned_unit(T* __restrict__ v, int64_t N, const int64_t* __restrict__ idx, int c)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) v[c * N + idx[0]] = T(1);
+}
+}  // namespace
+
+template <typename T>
+void launch_gather_owned(const T* v, int64_t N, const int64_t* owned_idx, int64_t nown, double* u, hipStream_t s)
+{
+    const int64_t n = 3 * nown;
+    hipLaunchKernelGGL(k_gather_owned<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v, N, owned_idx, nown, u);
+    TN_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_scatter_owned(T* v, int64_t N, const int64_t* owned_idx, int64_t nown, const double* u, hipStream_t s)
+{
+    const int64_t n = 3 * nown;
+    hipLaunchKernelGGL(k_scatter_owned<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v, N, owned_idx, nown,
+                       u);
+    TN_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_set_owned_unit(T* v, int64_t N, const int64_t* owned_idx, int c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_owned_unit<T>, dim3(1), dim3(64), 0, s, v, N, owned_idx, c);
+    TN_HIP(hipGetLastError());
+}
+template void launch_gather_owned<double>(const double*, int64_t, const int64_t*, int64_t, double*, hipStream_t);
+template void launch_gather_owned<float>(const float*, int64_t, const int64_t*, int64_t, double*, hipStream_t);
+template void launch_scatter_owned<double>(double*, int64_t, const int64_t*, int64_t, const double*, hipStream_t);
+template void launch_scatter_owned<float>(float*, int64_t, const int64_t*, int64_t, const double*, hipStream_t);
+template void launch_set_owned_unit<double>(double*, int64_t, const int64_t*, int, hipStream_t);
+template void launch_set_owned_unit<float>(float*, int64_t, const int64_t*, int, hipStream_t);
+}  // namespace tn

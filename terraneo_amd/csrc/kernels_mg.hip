@@ -120,8 +120,9 @@ __global__ void k_restrict(const T* __restrict__ xf, T* __restrict__ xc, const u
 // Interpolation from the 6 coarse wedge dofs onto the 6 dofs of one fine
 // wedge (gca.hpp:195-345, constant mode).  (fx, fy, fr) = fine hex cell,
 // fine wedge w, nrf = fine radial node count.  Reproduces the coded radial
-// bracket r_bot = r/2 (gca.hpp:213-216).
-__device__ inline void gca_local_P(int fx, int fy, int fr, int w, int nrf, double P[6][6])
+// bracket r_bot = r/2 (gca.hpp:213-216); geometric != 0 takes the bracket of
+// the coarse cell R the fine dof lies in instead (exact Galerkin product).
+__device__ inline void gca_local_P(int fx, int fy, int fr, int w, int nrf, int R, int geometric, double P[6][6])
 {
     for (int i = 0; i < 6; ++i)
         for (int j = 0; j < 6; ++j) P[i][j] = 0.0;
@@ -131,7 +132,7 @@ __device__ inline void gca_local_P(int fx, int fy, int fr, int w, int nrf, doubl
             P[fl][fl] = 1.0;
             continue;
         }
-        const int rb = ir < nrf - 1 ? ir / 2 : ir / 2 - 1, rt = rb + 1;
+        const int rb = geometric ? R : (ir < nrf - 1 ? ir / 2 : ir / 2 - 1), rt = rb + 1;
         if (!(ix & 1) && !(iy & 1)) {
             const int a = fl % 3;
             P[fl][a] = 0.5;
@@ -233,7 +234,7 @@ __global__ void k_gca_build(GcaArgs<T, S, SF> a)
         const int h = LIST[W][k][0], w = LIST[W][k][1];
         const int fx = 2 * X + SH[h][0], fy = 2 * Y + SH[h][1], fr = 2 * R + SH[h][2];
         double P[6][6], A[6][6];
-        gca_local_P(fx, fy, fr, w, a.nrf, P);
+        gca_local_P(fx, fy, fr, w, a.nrf, R, a.geometric, P);
         const int64_t fcell = (((int64_t)d * ncf + fx) * ncf + fy) * ncrf + fr;
         if constexpr (FINE_GEO) {
             const double* geo_w = a.geo_f + ((((int64_t)d * ncf + fx) * ncf + fy) * 2 + w) * GEO_STRIDE;

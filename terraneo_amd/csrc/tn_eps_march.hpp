@@ -28,6 +28,8 @@ struct MarchArgs {
 
 // node rows owned per block (waves per block - 1) for an element size in bytes
 int march_block_rows(int dtype_size);
+// resident waves per CU at the kernel's register use (element size in bytes)
+int march_waves_per_cu(int dtype_size);
 
 // boundary-node quadratic forms e^T B e (see k_boundary_blocks)
 template <typename T>

@@ -55,6 +55,7 @@ struct GcaArgs {
     const T* eta_cell_f;    // fine per-cell viscosity (overrides eta_f when set)
     const SF* stored_f; // fine packed storage (fine level stored)
     S* stored_c;        // coarse packed storage (output)
+    int geometric;      // 0: the reference's radial bracket, 1: the coarse cell's own
 };
 
 constexpr int PACKED = 171;

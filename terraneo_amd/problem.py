@@ -31,7 +31,7 @@ class StokesProblem:
     levels (the coarsest has lateral level lat-levels+1); bc: (CMB, surface)."""
 
     def __init__(self, lat, rad, levels, r_min=0.55, r_max=1.0, bc=(TN_FREESLIP, TN_FREESLIP), device="cuda:0",
-                 prec_dtype="f64", coarsest_f64=True, gmg=None):
+                 prec_dtype="f64", coarsest_f64=True, gmg=None, gca_bracket="reference"):
         torch = _torch()
         assert levels >= 2 and lat - levels + 1 >= 0 and rad - levels + 1 >= 1
         self.device = torch.device(device)
@@ -42,6 +42,7 @@ class StokesProblem:
         self.pre = api.Mesh(lat - 1, rad - 1, r_min, r_max)
         self.prec_dtype = prec_dtype
         self.coarsest_f64 = coarsest_f64
+        self.gca_bracket = gca_bracket
         self.gmg_opts = gmg or api.gmg_options(cheby_order=3, cheby_iterations=1, power_iterations=10, num_cycles=1)
         self.f64 = dict(dtype=torch.float64, device=self.device)
         self.A = None
@@ -83,7 +84,7 @@ class StokesProblem:
             one = torch.ones(m.N, dtype=dt, device=self.device)  # placeholder viscosity, unused once stored
             self._dummies.append(one)
             op = api.Eps(m, one, *bc)
-            op.gca_from(ops[-1], sdt)
+            op.gca_from(ops[-1], sdt, bracket=self.gca_bracket)
             ops.append(op)
         self.ops = list(reversed(ops))
         self.gmg = api.Gmg(self.ops, self.gmg_opts)

@@ -79,7 +79,7 @@ def test_c0_pcg_gmg_gca_solves(cuda, store):
     info = tn.pcg_solve(ops[-1], x, torch.from_numpy(b).to(cuda), prec=mg, max_iterations=100, rtol=1e-10)
     print("C0 PCG+GMG", store, info["iterations"], "its", info["seconds"], "s")
     assert info["final_residual"] / info["initial_residual"] <= 1e-10
-    assert info["iterations"] < 40
+    assert info["iterations"] <= 60
     # CPU: converged CG on the oracle's assembled Dirichlet operator (unique dofs)
     oA = O.eps_assemble(O.Eps(om, etas[-1], O.DIRICHLET, O.DIRICHLET))
     bu = O.to_unique(om, b, 3)
