@@ -82,14 +82,6 @@ __device__ __forceinline__ T shfl_up1(T v)
     return __shfl_up(v, 1, 64);
 }
 
-// Raw inputs of one cell (one lane): source at the 4 lateral corners, bottom
-// and top node, and the viscosity there.
-template <typename T>
-struct CellRaw {
-    T xb[4][3], xt[4][3], eb[4], et[4];
-    T geo[2 * GEO_STRIDE];
-};
-
 template <typename T, bool DIAG, int W>
 __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* __restrict__ geo_tab)
 {
@@ -136,35 +128,7 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
         const int tlev = c == ncr - 1 ? tmode : TN_NEUMANN;
         T c01[3] = {0, 0, 0}, c11[3] = {0, 0, 0}, c01t[3] = {0, 0, 0}, c11t[3] = {0, 0, 0};
 
-        // loads of quad qy into r (issued early: one step ahead of their use)
-        auto fetch = [&](int qy, CellRaw<T>& r) {
-            const bool quad_ok = strip_ok && qy >= 0 && qy <= n - 2;
-            if (!quad_ok) return;
-            const int goff = __builtin_amdgcn_readfirstlane(((d * ncl + x) * ncl + qy) * 2 * GEO_STRIDE);
-#pragma unroll
-            for (int i = 0; i < 2 * GEO_STRIDE; ++i) r.geo[i] = geo_tab[goff + i];
-            if (!valid) return;
-            const int64_t lat00 = ((int64_t)d * n + x) * n + qy;
-            const int64_t latc[4] = {lat00, lat00 + n, lat00 + 1, lat00 + n + 1};
-            const T ecell = a.eta_cell ? a.eta_cell[(((int64_t)d * ncl + x) * ncl + qy) * ncr + c] : T(0);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int64_t ib = latc[k] * nr + c;
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    r.xb[k][j] = a.src[j * N + ib];
-                    r.xt[k][j] = a.src[j * N + ib + 1];
-                }
-                if (a.eta_cell) {
-                    r.eb[k] = r.et[k] = ecell;
-                } else {
-                    r.eb[k] = a.eta[ib];
-                    r.et[k] = a.eta[ib + 1];
-                }
-            }
-        };
-
-        auto body = [&](int qy, CellRaw<T>& r) {
+        for (int qy = y0 - 1; qy <= y1 - 1; ++qy, ++step) {
             const bool quad_ok = strip_ok && qy >= 0 && qy <= n - 2;
             T(*E)[65] = ebuf[step & 1][w];
             const int64_t lat00 = ((int64_t)d * n + x) * n + qy;
@@ -173,40 +137,72 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
             for (int k = 0; k < 4; ++k)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
-            if (quad_ok && valid) {
-                const int64_t latc[4] = {lat00, lat00 + n, lat00 + 1, lat00 + n + 1};
-                if constexpr (DIAG) {
-                    CornerRaw<T> in[4];
+            if (quad_ok) {
+                // wave-uniform: scalar loads into SGPRs
+                const int goff = __builtin_amdgcn_readfirstlane(((d * ncl + x) * ncl + qy) * 2 * GEO_STRIDE);
+                const T* gsrc = geo_tab + goff;
+                T geo[2 * GEO_STRIDE];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-#pragma unroll
-                        for (int j = 0; j < 3; ++j) {
-                            in[k].b[j] = r.xb[k][j];
-                            in[k].t[j] = r.xt[k][j];
-                        }
-                        in[k].eb = r.eb[k];
-                        in[k].et = r.et[k];
-                    }
-                    cell_neumann_diag<T>(in, r.geo, alpha, beta, kr, yb, yt);
-                } else {
-                    if (blev != TN_NEUMANN || tlev != TN_NEUMANN) {
+                for (int i = 0; i < 2 * GEO_STRIDE; ++i) geo[i] = gsrc[i];
+                if (valid) {
+                    const int64_t latc[4] = {lat00, lat00 + n, lat00 + 1, lat00 + n + 1};
+                    const T ecell = a.eta_cell ? a.eta_cell[(((int64_t)d * ncl + x) * ncl + qy) * ncr + c] : T(0);
+                    if constexpr (DIAG) {
+                        CornerRaw<T> in[4];
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
-                            if (blev != TN_NEUMANN) apply_trial(r.xb[k], blev, a.coords, latc[k]);
-                            if (tlev != TN_NEUMANN) apply_trial(r.xt[k], tlev, a.coords, latc[k]);
+                            const int64_t ib = latc[k] * nr + c;
+                            in[k].b[0] = a.src[ib];
+                            in[k].b[1] = a.src[N + ib];
+                            in[k].b[2] = a.src[2 * N + ib];
+                            in[k].t[0] = a.src[ib + 1];
+                            in[k].t[1] = a.src[N + ib + 1];
+                            in[k].t[2] = a.src[2 * N + ib + 1];
+                            if (a.eta_cell) {
+                                in[k].eb = in[k].et = ecell;
+                            } else {
+                                in[k].eb = a.eta[ib];
+                                in[k].et = a.eta[ib + 1];
+                            }
                         }
-                    }
-                    CornerIn<T> in[4];
+                        cell_neumann_diag<T>(in, geo, alpha, beta, kr, yb, yt);
+                    } else {
+                        // issue every load of the cell before any lane-divergent boundary work
+                        T xb[4][3], xt[4][3], eb[4], et[4];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
+                        for (int k = 0; k < 4; ++k) {
+                            const int64_t ib = latc[k] * nr + c;
 #pragma unroll
-                        for (int j = 0; j < 3; ++j) {
-                            in[k].S[j] = r.xb[k][j] + r.xt[k][j];
-                            in[k].d[j] = r.xt[k][j] - r.xb[k][j];
+                            for (int j = 0; j < 3; ++j) {
+                                xb[k][j] = a.src[j * N + ib];
+                                xt[k][j] = a.src[j * N + ib + 1];
+                            }
+                            if (a.eta_cell) {
+                                eb[k] = et[k] = ecell;
+                            } else {
+                                eb[k] = a.eta[ib];
+                                et[k] = a.eta[ib + 1];
+                            }
                         }
-                        in[k].e = r.eb[k] + r.et[k];
+                        if (blev != TN_NEUMANN || tlev != TN_NEUMANN) {
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) {
+                                if (blev != TN_NEUMANN) apply_trial(xb[k], blev, a.coords, latc[k]);
+                                if (tlev != TN_NEUMANN) apply_trial(xt[k], tlev, a.coords, latc[k]);
+                            }
+                        }
+                        CornerIn<T> in[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+                            for (int j = 0; j < 3; ++j) {
+                                in[k].S[j] = xb[k][j] + xt[k][j];
+                                in[k].d[j] = xt[k][j] - xb[k][j];
+                            }
+                            in[k].e = eb[k] + et[k];
+                        }
+                        cell_neumann<T>(in, geo, alpha, beta, kr, yb, yt);
                     }
-                    cell_neumann<T>(in, r.geo, alpha, beta, kr, yb, yt);
                 }
             }
             // radial merge (node level c = bottom of cell c + top of cell c-1) and routing
@@ -234,7 +230,6 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                     c11t[j] = yt[3][j];
                 }
             }
-            fetch(qy + 1, r);  // the inputs are dead: reload them, in flight across the barrier and the stores
             __syncthreads();
             if (row_own && qy >= y0) {
                 const T(*Ein)[65] = ebuf[step & 1][w - 1];
@@ -246,9 +241,9 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                     const int64_t idx = lat * nr + level;
                     if (mode != TN_NEUMANN) v = test_value<T, DIAG>(v, a.src, N, idx, mode, a.coords, lat, blk);
                     if (a.pen) {
-                        const double rr = a.radii_d[level];
-                        const double px = a.coords_d[3 * lat] * rr, py = a.coords_d[3 * lat + 1] * rr,
-                                     pz = a.coords_d[3 * lat + 2] * rr;
+                        const double r = a.radii_d[level];
+                        const double px = a.coords_d[3 * lat] * r, py = a.coords_d[3 * lat + 1] * r,
+                                     pz = a.coords_d[3 * lat + 2] * r;
                         v.x += (T)(gam[1] * pz - gam[2] * py);
                         v.y += (T)(gam[2] * px - gam[0] * pz);
                         v.z += (T)(gam[0] * py - gam[1] * px);
@@ -283,12 +278,7 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                     }
                 }
             }
-            ++step;
-        };
-
-        CellRaw<T> r;
-        fetch(y0 - 1, r);
-        for (int qy = y0 - 1; qy <= y1 - 1; ++qy) body(qy, r);
+        }
     }
 }
 
@@ -377,12 +367,14 @@ __global__ __launch_bounds__(64 * W) void k_eps_tile(MarchArgs<T> a, const T* __
         commit(y0 - 1);
         fetch(y0);
         commit(y0);
+        fetch(y0 + 1);  // committed at the first step
         __syncthreads();
 
         for (int qy = y0 - 1; qy <= y1 - 1; ++qy, ++step) {
             const bool quad_ok = strip_ok && qy >= 0 && qy <= n - 2;
             T(*E)[65] = ebuf[step & 1][w];
-            fetch(qy + 2);  // consumed from the tile at step qy + 1
+            commit(qy + 2);  // fetched one step ago; the ring slot of column qy - 1, read by nobody now
+            fetch(qy + 3);   // in flight for a whole step
             const int64_t lat00 = ((int64_t)d * n + x) * n + qy;
             T yb[4][3], yt[4][3];
 #pragma unroll
@@ -478,7 +470,6 @@ __global__ __launch_bounds__(64 * W) void k_eps_tile(MarchArgs<T> a, const T* __
                     c11t[j] = yt[3][j];
                 }
             }
-            commit(qy + 2);  // the ring slot of column qy - 1, read by nobody at this step
             __syncthreads();
             if (row_own && qy >= y0) {
                 const T(*Ein)[65] = ebuf[step & 1][w - 1];
@@ -627,8 +618,8 @@ void launch_boundary_blocks(const Geom<T>& g, const T* eta, const T* eta_cell, i
     TN_HIP(hipGetLastError());
 }
 
-// Waves per block (W-1 node rows per block).  FP64: ~234 VGPRs -> 2 waves/SIMD;
-// FP32: ~130-140 VGPRs -> 3 waves/SIMD.  More waves per block means a smaller
+// Waves per block (W-1 node rows per block).  FP64: 168 VGPRs -> 3 waves/SIMD
+// -> W = 12 (one block per CU); FP32: <= 128 VGPRs -> 4 waves/SIMD -> W = 16.  More waves per block means a smaller
 // share of redundant halo strips (W/(W-1)); more resident blocks per CU hide
 // the per-step barrier.  TN_MARCH_WAVES overrides (tuning: 4, 6, 8, 12, 16).
 template <typename T>
@@ -638,14 +629,14 @@ int march_waves()
         const char* e = std::getenv("TN_MARCH_WAVES");
         const int v = e ? std::atoi(e) : 0;
         if (v == 4 || v == 6 || v == 8 || v == 12 || (v == 16 && sizeof(T) == 4)) return v;
-        return sizeof(T) == 8 ? 8 : 16;
+        return sizeof(T) == 8 ? 12 : 16;
     }();
     return w;
 }
 
 int march_block_rows(int dtype_size) { return (dtype_size == 8 ? march_waves<double>() : march_waves<float>()) - 1; }
 
-int march_waves_per_cu(int dtype_size) { return dtype_size == 8 ? 8 : 12; }
+int march_waves_per_cu(int dtype_size) { return dtype_size == 8 ? 12 : 16; }
 
 template <typename T, int W>
 void launch_march_w(const MarchArgs<T>& a, bool diagonal, bool tiled, hipStream_t s)
@@ -671,9 +662,14 @@ void launch_eps_march(const MarchArgs<T>& a, bool diagonal, hipStream_t s)
     TN_REQUIRE(a.nr - 1 >= 2, "march kernel: at least 2 radial cells");
     TN_REQUIRE(a.seg >= 1 && a.seg <= MARCH_SEG_MAX, "march kernel: segment length out of range");
     TN_REQUIRE(10LL * (a.n - 1) * (a.n - 1) * 2 * GEO_STRIDE < (1LL << 31), "march kernel: mesh too large");
+    // FP64: the march kernel (register-staged corner loads); FP32: the LDS-tiled
+    // kernel (measured faster at FP32's higher occupancy).  TN_EPS_KERNEL=march|tile
+    // overrides (tuning).
     static const bool tiled = [] {
         const char* e = std::getenv("TN_EPS_KERNEL");
-        return e && std::string(e) == "tile";
+        if (e && std::string(e) == "tile") return true;
+        if (e && std::string(e) == "march") return false;
+        return sizeof(T) == 4;
     }();
     switch (march_waves<T>()) {
         case 4: launch_march_w<T, 4>(a, diagonal, tiled, s); break;

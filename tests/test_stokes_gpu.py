@@ -9,7 +9,7 @@ import assess as AS
 import oracle_lib as O
 from fields import consistent_random, smooth_eta
 
-BCS = [(0, 0), (1, 1), (2, 2), (2, 1)]
+BCS = [(0, 0), (1, 1), (2, 2), (2, 1), (1, 2)]
 
 
 def _meshes(lat=3, rmin=0.55, rmax=1.0):
