@@ -76,6 +76,14 @@ typedef struct tn_krylov_options {
     double rtol;
     double atol;
     int restart; /* FGMRES only */
+    /* tn_stokes_solve only.  0: solve K x = b as given (reference behaviour).
+     * 1: solve in the pressure space orthogonal to constants: the operator
+     * becomes Pi K Pi with Pi removing the owned-node mean of the pressure
+     * block, b and the result are projected the same way.  This is the
+     * system bordered by the mean-pressure constraint; it drops the
+     * near-null constant-pressure mode that free-slip shells with nodal
+     * normals leave in K (B^T 1 is then small but not zero). */
+    int pressure_mean_free;
 } tn_krylov_options;
 
 typedef struct tn_solve_info {

@@ -24,7 +24,8 @@ class GmgOptions(C.Structure):
 
 
 class KrylovOptions(C.Structure):
-    _fields_ = [("max_iterations", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("restart", C.c_int)]
+    _fields_ = [("max_iterations", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("restart", C.c_int),
+                ("pressure_mean_free", C.c_int)]
 
 
 class SolveInfo(C.Structure):

@@ -280,7 +280,7 @@ def _info_dict(info, hist):
 
 
 def pcg_solve(A, x, b, prec=None, max_iterations=100, rtol=1e-8, atol=1e-30, stream=None):
-    o = KrylovOptions(max_iterations, rtol, atol, 30)
+    o = KrylovOptions(max_iterations, rtol, atol, 30, 0)
     info = SolveInfo()
     hist = np.zeros(max_iterations + 2)
     check(lib().tn_pcg_solve(A.h, prec.h if prec is not None else None, _ptr(x), _ptr(b), C.byref(o), C.byref(info),
@@ -289,8 +289,8 @@ def pcg_solve(A, x, b, prec=None, max_iterations=100, rtol=1e-8, atol=1e-30, str
 
 
 def stokes_solve(K, method, prec_u, schur_diag, u, p, bu, bp, max_iterations=100, rtol=1e-8, atol=1e-30,
-                 restart=30, stream=None):
-    o = KrylovOptions(max_iterations, rtol, atol, restart)
+                 restart=30, pressure_mean_free=False, stream=None):
+    o = KrylovOptions(max_iterations, rtol, atol, restart, int(bool(pressure_mean_free)))
     info = SolveInfo()
     hist = np.zeros(max_iterations + 2)
     check(lib().tn_stokes_solve(K.h, method, prec_u.h, _ptr(schur_diag), _ptr(u), _ptr(p), _ptr(bu), _ptr(bp),

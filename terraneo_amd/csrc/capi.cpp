@@ -97,19 +97,19 @@ int device_cus()
     return cache[dev];
 }
 
-// Segment length (node columns per block) of the march kernel: one block is
-// resident per CU, so the kernel takes ceil(blocks / CUs) rounds of
-// (seg + 1) quad columns each; pick the segment that minimises that product.
-// TN_MARCH_SEG overrides (tuning).
-int march_seg(int n, int dtype_size)
+// Segment length (node columns per block) of a march kernel with `own` node
+// rows per block and `waves_per_cu` resident waves per CU: the kernel takes
+// ceil(blocks / resident blocks) rounds of (seg + 1) quad columns each; pick
+// the segment that minimises that product.  TN_MARCH_SEG overrides (tuning).
+}  // namespace
+int tn::march_seg_for(int n, int own, int waves_per_cu)
 {
     static const int env = [] {
         const char* e = std::getenv("TN_MARCH_SEG");
         return e ? std::atoi(e) : 0;
     }();
     if (env > 0) return std::min(std::min(env, n), tn::MARCH_SEG_MAX);
-    const int own = tn::march_block_rows(dtype_size);
-    const int64_t per_cu = std::max(1, tn::march_waves_per_cu(dtype_size) / (own + 1));
+    const int64_t per_cu = std::max(1, waves_per_cu / (own + 1));
     const int64_t nxb = (n + own - 1) / own, cus = device_cus() * per_cu;
     int best = std::min(n, tn::MARCH_SEG_MAX);
     int64_t best_cost = -1;
@@ -122,6 +122,12 @@ int march_seg(int n, int dtype_size)
         }
     }
     return best;
+}
+namespace {
+
+int march_seg(int n, int dtype_size)
+{
+    return tn::march_seg_for(n, tn::march_block_rows(dtype_size), tn::march_waves_per_cu(dtype_size));
 }
 
 template <typename T>

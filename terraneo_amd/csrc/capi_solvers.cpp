@@ -52,6 +52,52 @@ BVec vec2(const Field& u, const Field& p)
     return v;
 }
 
+// Removes the owned-node mean of a scalar (pressure) field in place:
+// p -= (sum over owned nodes of p) / (number of owned nodes), the orthogonal
+// projection onto the complement of the constants in the owned inner product.
+// Two launches, no host synchronisation.
+struct MeanFree {
+    MeshLevel* m;
+    int dtype;
+    VecStore st;
+    BVec ones;
+    DevBuf<double> sum;
+    double inv_count = 0.0;
+    MeanFree(Ctx& c, const Field& like) : m(like.m), dtype(like.dtype), sum(1)
+    {
+        ones = st.make(vec1(like));
+        v_fill(c, ones, 1.0);
+        inv_count = 1.0 / v_dot(c, ones, ones);
+    }
+    void project(Ctx& c, void* p)
+    {
+        launch_fill<double>(sum.p, 1, 0.0, c.s);
+        if (dtype == TN_F64) {
+            launch_dot_owned<double>(*m, 1, (const double*)p, (const double*)ones.b[0].p, sum.p, c.s);
+            launch_sub_scaled<double>((double*)p, m->N, sum.p, inv_count, c.s);
+        } else {
+            launch_dot_owned<float>(*m, 1, (const float*)p, (const float*)ones.b[0].p, sum.p, c.s);
+            launch_sub_scaled<float>((float*)p, m->N, sum.p, inv_count, c.s);
+        }
+    }
+};
+
+// Pi K Pi with Pi the mean removal of the pressure block (the Stokes system
+// bordered by the mean-pressure constraint).  The input's pressure is
+// projected into a private copy, so Krylov basis vectors stay untouched.
+struct MeanFreeStokes : LinOp {
+    LinOp* K;
+    MeanFree* mf;
+    BVec ptmp;  // one scalar block
+    void apply(Ctx& c, const BVec& x, const BVec& y) override
+    {
+        v_assign(c, ptmp, vec1(x.b[1]));
+        mf->project(c, ptmp.b[0].p);
+        K->apply(c, vec2(x.b[0], ptmp.b[0]), y);
+        mf->project(c, y.b[1].p);
+    }
+};
+
 struct EpsOp : LinOp {
     tn_eps* op;
     explicit EpsOp(tn_eps* o) : op(o) {}
@@ -309,6 +355,22 @@ int tn_stokes_solve(tn_stokes* K, int method, tn_gmg* prec_u, const void* schur_
         pu = bridge.get();
     }
     EpsOp A11(K->A);
+    // optional mean-free pressure: project b and the initial guess, solve with Pi K Pi
+    std::unique_ptr<MeanFree> mf;
+    MeanFreeStokes Kmf;
+    LinOp* Kuse = &Kop;
+    BVec Buse = B;
+    if (o->pressure_mean_free) {
+        mf.reset(new MeanFree(c, fp));
+        Kmf.K = &Kop;
+        Kmf.mf = mf.get();
+        Kmf.ptmp = st.make(vec1(fp));
+        Kuse = &Kmf;
+        Buse = st.make(X);
+        v_assign(c, Buse, B);
+        mf->project(c, Buse.b[1].p);
+        mf->project(c, X.b[1].p);
+    }
     KrylovParams kp{o->max_iterations, o->rtol, o->atol, o->restart};
     IterInfo it;
     int restarts = 0, skipped = 0;
@@ -319,7 +381,7 @@ int tn_stokes_solve(tn_stokes* K, int method, tn_gmg* prec_u, const void* schur_
         std::vector<BVec> t;
         for (int i = 0; i < 7; ++i) t.push_back(st.make(X));
         PMINRES solver(kp, t, &P);
-        solver.solve(c, Kop, X, B);
+        solver.solve(c, *Kuse, X, Buse);
         it = solver.info;
     } else {
         TN_REQUIRE(method == TN_FGMRES_BLOCKTRI || method == TN_FGMRES_BLOCKDIAG, "unknown Stokes solver method");
@@ -343,11 +405,12 @@ int tn_stokes_solve(tn_stokes* K, int method, tn_gmg* prec_u, const void* schur_
         else
             P.reset(new BlockDiagonal(&A11, pu, &null, &schur));
         FGMRES solver(kp, t, P.get(), true);
-        solver.solve(c, Kop, X, B);
+        solver.solve(c, *Kuse, X, Buse);
         it = solver.info;
         restarts = solver.restarts_breakdown + solver.restarts_stagnation;
         skipped = solver.prec_skipped;
     }
+    if (mf) mf->project(c, X.b[1].p);
     TN_HIP(hipStreamSynchronize(c.s));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     fill_info(info, it, secs, history, history_cap);

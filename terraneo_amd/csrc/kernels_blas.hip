@@ -78,6 +78,16 @@ __global__ __launch_bounds__(256) void k_dot_owned(const T* __restrict__ a, cons
     block_accumulate(s, out);
 }
 
+// y[i] -= scale * sum[0] (sum: a device scalar from a preceding reduction)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sub_scaled(T* __restrict__ y, int64_t count, const double* __restrict__ sum,
+                                                    double scale)
+{
+    const T v = (T)(scale * sum[0]);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] -= v;
+}
+
 // One thread per (duplicated lateral group, radial index), r fastest.
 template <typename T>
 __global__ void k_interface_sum(const int32_t* __restrict__ off, const int32_t* __restrict__ mem, int ngroups,
@@ -268,6 +278,13 @@ void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, dou
 }
 
 template <typename T>
+void launch_sub_scaled(T* y, int64_t count, const double* sum_dev, double scale, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sub_scaled<T>, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, y, count, sum_dev, scale);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
 void launch_interface_sum(const MeshLevel& m, int ncomp, T* v, hipStream_t s)
 {
     const int ng = m.ngroups();
@@ -353,6 +370,7 @@ template void launch_cast<float, double>(float*, const double*, int64_t, hipStre
     template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                                   \
     template void launch_lincomb<T>(int64_t, T*, int, const double*, const T* const*, double, hipStream_t);     \
     template void launch_dot_owned<T>(const MeshLevel&, int, const T*, const T*, double*, hipStream_t);          \
+    template void launch_sub_scaled<T>(T*, int64_t, const double*, double, hipStream_t);                         \
     template void launch_interface_sum<T>(const MeshLevel&, int, T*, hipStream_t);                               \
     template void launch_penalty_moment<T>(const MeshLevel&, const T*, double*, hipStream_t);                    \
     template void launch_penalty_init<T>(const MeshLevel&, const double*, const double*, T*, bool, hipStream_t); \

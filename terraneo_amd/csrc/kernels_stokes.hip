@@ -17,11 +17,16 @@
 // p_H(q) = sum_i N^H_i(q) p_i.  Dirichlet layers drop their velocity dofs,
 // free-slip layers project them tangentially (trial side for B, test side
 // for B^T), which is the masked / rotated element matrix of the reference.
+#include "tn_cell.hpp"
+#include "tn_eps_march.hpp"
 #include "tn_internal.hpp"
 #include "tn_stokes.hpp"
 
 namespace tn {
 namespace {
+
+using cellops::dot3;
+using cellops::V3;
 
 __device__ constexpr int WOFF[2][6][3] = {{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}},
                                          {{1, 1, 0}, {0, 1, 0}, {1, 0, 0}, {1, 1, 1}, {0, 1, 1}, {1, 0, 1}}};
@@ -40,70 +45,10 @@ __device__ inline double shape_lat(int k, double xi, double eta)
 }
 __device__ inline double shape_rad(int k, double zeta) { return k < 3 ? 0.5 * (1 - zeta) : 0.5 * (1 + zeta); }
 
-__device__ inline void grad_shape(int k, const double q[3], double g[3])
-{
-    const int m = k % 3;
-    const double nr = shape_rad(k, q[2]);
-    g[0] = (m == 0 ? -1.0 : (m == 1 ? 1.0 : 0.0)) * nr;
-    g[1] = (m == 0 ? -1.0 : (m == 2 ? 1.0 : 0.0)) * nr;
-    g[2] = shape_lat(k, q[0], q[1]) * (k < 3 ? -0.5 : 0.5);
-}
-
-// coarse shape i of the wedge containing the fine wedge (frw: fine radial
-// index r%2, flw: fine lateral index 0..3), integrands.hpp:103-190
-__device__ inline double shape_coarse(int i, int frw, int flw, const double q[3])
-{
-    const double xi = q[0], eta = q[1], z = q[2];
-    double lat;
-    switch (i % 3) {
-    case 0: lat = flw == 0 ? 1.0 - 0.5 * (xi + eta) : (flw == 3 ? 0.5 * (xi + eta) : 0.5 - 0.5 * (xi + eta)); break;
-    case 1: lat = flw == 0 ? 0.5 * xi : (flw == 1 ? 0.5 * xi + 0.5 : (flw == 2 ? 0.5 * xi : 0.5 - 0.5 * xi)); break;
-    default: lat = flw == 0 ? 0.5 * eta : (flw == 1 ? 0.5 * eta : (flw == 2 ? 0.5 * eta + 0.5 : 0.5 - 0.5 * eta)); break;
-    }
-    const double rad = i < 3 ? (frw == 0 ? 0.25 * (3 - z) : 0.25 * (1 - z)) : (frw == 0 ? 0.25 * (1 + z) : 0.25 * (3 + z));
-    return lat * rad;
-}
-
-__device__ inline int fine_lateral_wedge_idx(int x, int y, int w)
-{
-    // kernel_helpers.hpp:600-608
-    constexpr int ind[2][2][2] = {{{0, 1}, {2, 3}}, {{3, 2}, {1, 0}}};
-    return ind[w][y & 1][x & 1];
-}
-
 struct WedgeGeo {
     double P[3][3];  // lateral vertices (unit sphere)
     double r1, r2;
 };
-
-// J^{-T} grad N_j at quad point q and |det J| (integrands.hpp:623-670)
-__device__ inline double wedge_qp(const WedgeGeo& G, const double q[3], double g[6][3])
-{
-    const double r = G.r1 + 0.5 * (G.r2 - G.r1) * (1 + q[2]);
-    const double gr = 0.5 * (G.r2 - G.r1);
-    double J[3][3];
-    for (int a = 0; a < 3; ++a) {
-        J[a][0] = (G.P[1][a] - G.P[0][a]) * r;
-        J[a][1] = (G.P[2][a] - G.P[0][a]) * r;
-        J[a][2] = ((1 - q[0] - q[1]) * G.P[0][a] + q[0] * G.P[1][a] + q[1] * G.P[2][a]) * gr;
-    }
-    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-    const double id = 1.0 / det;
-    const double K[3][3] = {{c00 * id, c01 * id, c02 * id},
-                            {(J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id,
-                             (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id},
-                            {(J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id,
-                             (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
-    for (int j = 0; j < 6; ++j) {
-        double gr3[3];
-        grad_shape(j, q, gr3);
-        for (int a = 0; a < 3; ++a) g[j][a] = K[a][0] * gr3[0] + K[a][1] * gr3[1] + K[a][2] * gr3[2];
-    }
-    return fabs(det);
-}
 
 __device__ inline void load_wedge(const double* coords, const double* radii, int n, int d, int x, int y, int rc, int w,
                                   WedgeGeo& G)
@@ -118,119 +63,487 @@ __device__ inline void load_wedge(const double* coords, const double* radii, int
     G.r2 = radii[rc + 1];
 }
 
-__device__ inline void unit_normal(const double* coords, int64_t lat, double nn[3])
+// ---------------------------------------------------------------------------
+// B and B^T by per-wedge moments.
+//
+// A fine wedge with lateral unit-sphere vertices P0, P1, P2 (a = P1 - P0,
+// b = P2 - P0) between radii r1 < r2 is x(xi, eta, zeta) = r(zeta) c(xi, eta)
+// with c = P0 + xi a + eta b, r = r1 + g (1 + zeta), g = (r2 - r1) / 2.  Its
+// Jacobian [r a, r b, g c] has det J = r^2 g D, D = det[a, b, P0], constant
+// over the triangle, and for a shape N
+//   |det J| grad N = sgn(D) [ r g (N_xi (v1 - xi nv) + N_eta (v3 - eta nv))
+//                             + r^2 N_zeta nv ],
+//   v1 = b x P0,  v3 = P0 x a,  nv = a x b.
+// With the wedge shapes N_(m,t) = L_m(xi, eta) R_t(zeta) (L = 1 - xi - eta,
+// xi, eta; R = (1 -/+ zeta) / 2) the divergence of u_h is, per node level t,
+//   |det J| div u_h = sgn(D) sum_t [ r g R_t (alpha_t - xi bx_t - eta be_t)
+//                                    + r^2 R_t' (sigma_t + xi bx_t + eta be_t) ]
+//   du1 = u_1t - u_0t, du2 = u_2t - u_0t,
+//   alpha_t = v1.du1 + v3.du2, sigma_t = nv.u_0t, bx_t = nv.du1, be_t = nv.du2.
+// Against a coarse shape lat_il(xi, eta) rad_ir(zeta) (integrands.hpp:103-190)
+// the integrand has lateral degree <= 2 and radial degree <= 3, which the
+// 6-point Felippa rule of the reference (3-point triangle x 2-point Gauss)
+// integrates exactly, so the quadrature sum is the moment formula
+//   (B u)_i = -sgn(D) [ P0_ir M0_il + Px_ir Mx_il + Pe_ir Me_il ]
+//   P0_ir = sum_t alpha_t A_(ir,t) + sigma_t C_(ir,t)
+//   Px_ir = sum_t bx_t (C - A)_(ir,t),   Pe_ir = sum_t be_t (C - A)_(ir,t)
+// with the lateral moments M0, Mx, Me = int lat_il (1, xi, eta) over the
+// reference triangle and the radial moments A_(ir,t) = int rad_ir r g R_t,
+// C_(ir,t) = R_t' int rad_ir r^2 over [-1, 1].  B^T is its transpose:
+// with q0_ir = sum_il p_i M0_il (qx, qe likewise) and per level t
+//   X_t = sum_ir A q0, Y_t = sum_ir C q0, Zx_t = sum_ir (C - A) qx,
+//   Ze_t = sum_ir (C - A) qe,
+// the node values are -sgn(D) times
+//   (0, t): -X_t (v1 + v3) + (Y_t - Zx_t - Ze_t) nv
+//   (1, t):  X_t v1 + Zx_t nv
+//   (2, t):  X_t v3 + Ze_t nv.
+// About 100 operations per fine wedge either way (the per-quadrature-point
+// Jacobian inversion of the reference costs ~1700).
+
+__device__ constexpr double GAUSS2 = 0.57735026918962576451;  // 1/sqrt(3)
+
+// coarse lateral shape il on fine lateral wedge flw: alpha + beta xi + gamma eta
+// (integrands.hpp:103-190; flw from kernel_helpers.hpp:600-608)
+__host__ __device__ constexpr double coarse_lat_coef(int flw, int il, int k)
 {
-    const double* c = coords + 3 * lat;
-    const double inv = 1.0 / sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-    nn[0] = c[0] * inv;
-    nn[1] = c[1] * inv;
-    nn[2] = c[2] * inv;
+    constexpr double C[4][3][3] = {{{1.0, -0.5, -0.5}, {0.0, 0.5, 0.0}, {0.0, 0.0, 0.5}},
+                                   {{0.5, -0.5, -0.5}, {0.5, 0.5, 0.0}, {0.0, 0.0, 0.5}},
+                                   {{0.5, -0.5, -0.5}, {0.0, 0.5, 0.0}, {0.5, 0.0, 0.5}},
+                                   {{0.0, 0.5, 0.5}, {0.5, -0.5, 0.0}, {0.5, 0.0, -0.5}}};
+    return C[flw][il][k];
 }
 
-// B: one thread per fine cell, atomics into the coarse pressure field.
-template <typename T>
-__global__ __launch_bounds__(128) void k_div(DivArgs<T> a)
+// int over the reference triangle of (alpha + beta xi + gamma eta) * {1, xi, eta}
+struct LatMoments {
+    double m[4][3][3];  // [flw][il][M0, Mx, Me]
+};
+__host__ __device__ constexpr LatMoments make_lat_moments()
 {
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ncr = a.nrf - 1, ncl = a.nf - 1;
-    const int64_t ncell = 10LL * ncl * ncl * ncr;
-    if (tid >= ncell) return;
-    const int rc = (int)(tid % ncr);
-    int64_t lat = tid / ncr;
-    const int y = (int)(lat % ncl);
-    lat /= ncl;
-    const int x = (int)(lat % ncl);
-    const int d = (int)(lat / ncl);
-    const int64_t Nf = 10LL * a.nf * a.nf * a.nrf;
-    const int bmode = rc == 0 ? a.bc_cmb : TN_NEUMANN;
-    const int tmode = rc == ncr - 1 ? a.bc_surf : TN_NEUMANN;
-    const int frw = rc & 1;
-    for (int w = 0; w < 2; ++w) {
-        WedgeGeo G;
-        load_wedge(a.coords, a.radii, a.nf, d, x, y, rc, w, G);
-        double u[6][3];
-        for (int j = 0; j < 6; ++j) {
-            const int64_t lj = ((int64_t)d * a.nf + x + WOFF[w][j][0]) * a.nf + y + WOFF[w][j][1];
-            const int64_t ij = lj * a.nrf + rc + WOFF[w][j][2];
-            for (int c = 0; c < 3; ++c) u[j][c] = (double)a.u[c * Nf + ij];
-            const int mode = j < 3 ? bmode : tmode;
-            if (mode == TN_DIRICHLET) {
-                u[j][0] = u[j][1] = u[j][2] = 0.0;
-            } else if (mode == TN_FREESLIP) {
-                double nn[3];
-                unit_normal(a.coords, lj, nn);
-                const double un = nn[0] * u[j][0] + nn[1] * u[j][1] + nn[2] * u[j][2];
-                for (int c = 0; c < 3; ++c) u[j][c] -= un * nn[c];
-            }
+    LatMoments r{};
+    for (int f = 0; f < 4; ++f)
+        for (int i = 0; i < 3; ++i) {
+            const double al = coarse_lat_coef(f, i, 0), be = coarse_lat_coef(f, i, 1), ga = coarse_lat_coef(f, i, 2);
+            r.m[f][i][0] = al / 2 + (be + ga) / 6;
+            r.m[f][i][1] = al / 6 + be / 12 + ga / 24;
+            r.m[f][i][2] = al / 6 + be / 24 + ga / 12;
         }
-        const int flw = fine_lateral_wedge_idx(x, y, w);
-        double out[6] = {0, 0, 0, 0, 0, 0};
-        for (int q = 0; q < 6; ++q) {
-            double g[6][3];
-            const double adet = wedge_qp(G, Q6P[q], g);
-            double div = 0.0;
-            for (int j = 0; j < 6; ++j) div += g[j][0] * u[j][0] + g[j][1] * u[j][1] + g[j][2] * u[j][2];
-            const double f = -Q6W * adet * div;
-            for (int i = 0; i < 6; ++i) out[i] += f * shape_coarse(i, frw, flw, Q6P[q]);
+    return r;
+}
+__device__ constexpr LatMoments LATM = make_lat_moments();
+
+__device__ inline int fine_lateral_wedge_idx(int xp, int yp, int w)
+{
+    // kernel_helpers.hpp:600-608 (xp, yp: parities of the fine quad indices)
+    constexpr int ind[2][2][2] = {{{0, 1}, {2, 3}}, {{3, 2}, {1, 0}}};
+    return ind[w][yp][xp];
+}
+
+template <typename T>
+struct RadMoments {
+    T A[2][2];  // [ir][t] int rad_ir r g R_t
+    T C[2][2];  // [ir][t] R_t' int rad_ir r^2
+};
+
+// fine radial cell (r1, r2) in position frw (0 lower, 1 upper) of its coarse
+// cell; scale 0 zeroes every moment (lanes past the last cell)
+template <typename T>
+__device__ inline RadMoments<T> radial_moments(double r1, double r2, int frw, double scale)
+{
+    const double g = 0.5 * (r2 - r1);
+    double A[2][2] = {{0, 0}, {0, 0}}, rr[2] = {0, 0};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const double z = s ? GAUSS2 : -GAUSS2;
+        const double r = r1 + g * (1 + z);
+        const double R[2] = {0.5 * (1 - z), 0.5 * (1 + z)};
+        const double rad[2] = {frw ? 0.25 * (1 - z) : 0.25 * (3 - z), frw ? 0.25 * (3 + z) : 0.25 * (1 + z)};
+#pragma unroll
+        for (int ir = 0; ir < 2; ++ir) {
+            A[ir][0] += rad[ir] * r * g * R[0];
+            A[ir][1] += rad[ir] * r * g * R[1];
+            rr[ir] += rad[ir] * r * r;
         }
-        for (int i = 0; i < 6; ++i) {
-            const int64_t ic = (((int64_t)d * a.nc + x / 2 + WOFF[w][i][0]) * a.nc + y / 2 + WOFF[w][i][1]) * a.nrc +
-                               rc / 2 + WOFF[w][i][2];
-            unsafeAtomicAdd(a.p + ic, (T)out[i]);
-        }
+    }
+    RadMoments<T> m;
+#pragma unroll
+    for (int ir = 0; ir < 2; ++ir) {
+        m.A[ir][0] = (T)(scale * A[ir][0]);
+        m.A[ir][1] = (T)(scale * A[ir][1]);
+        m.C[ir][0] = (T)(-0.5 * scale * rr[ir]);
+        m.C[ir][1] = (T)(0.5 * scale * rr[ir]);
+    }
+    return m;
+}
+
+template <typename T>
+struct LatWedge {
+    V3<T> v1, v3, nv;
+    T sg;  // -sgn(D): the sign of B folded in
+};
+
+template <typename T>
+__device__ inline V3<T> cross3(const V3<T>& a, const V3<T>& b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+template <typename T>
+__device__ inline LatWedge<T> lat_wedge(const V3<T>& P0, const V3<T>& P1, const V3<T>& P2)
+{
+    const V3<T> a{P1.x - P0.x, P1.y - P0.y, P1.z - P0.z}, b{P2.x - P0.x, P2.y - P0.y, P2.z - P0.z};
+    LatWedge<T> L;
+    L.v1 = cross3(b, P0);
+    L.v3 = cross3(P0, a);
+    L.nv = cross3(a, b);
+    L.sg = dot3(a, L.v1) > T(0) ? T(-1) : T(1);
+    return L;
+}
+
+template <typename T>
+__device__ inline V3<T> load_coord(const double* coords, int64_t lat)
+{
+    return {(T)coords[3 * lat], (T)coords[3 * lat + 1], (T)coords[3 * lat + 2]};
+}
+
+template <typename T>
+__device__ inline V3<T> unit_of(const V3<T>& c)
+{
+    const T inv = T(1) / sqrt(dot3(c, c));
+    return {c.x * inv, c.y * inv, c.z * inv};
+}
+
+// trial-side boundary treatment of a shell node value: Dirichlet drops it,
+// free-slip keeps its tangential part (normal: the node's radial direction)
+template <typename T>
+__device__ inline void trial_bc(V3<T>& u, int mode, const V3<T>& P)
+{
+    if (mode == TN_DIRICHLET) {
+        u = {T(0), T(0), T(0)};
+    } else if (mode == TN_FREESLIP) {
+        const V3<T> nn = unit_of(P);
+        const T un = dot3(nn, u);
+        u = {u.x - un * nn.x, u.y - un * nn.y, u.z - un * nn.z};
     }
 }
 
-// B^T: one thread per fine cell, atomics into the fine velocity field.
+// (B u) contributions of one fine wedge to its six coarse dofs:
+// out[il][ir] += ..., u0/u1/u2 at node levels t = 0, 1
 template <typename T>
-__global__ __launch_bounds__(128) void k_grad(GradArgs<T> a)
+__device__ inline void div_wedge(const LatWedge<T>& L, const T (*M)[3], const RadMoments<T>& rm, const V3<T> u0[2],
+                                 const V3<T> u1[2], const V3<T> u2[2], T out[3][2])
 {
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ncr = a.nrf - 1, ncl = a.nf - 1;
-    const int64_t ncell = 10LL * ncl * ncl * ncr;
-    if (tid >= ncell) return;
-    const int rc = (int)(tid % ncr);
-    int64_t lat = tid / ncr;
-    const int y = (int)(lat % ncl);
-    lat /= ncl;
-    const int x = (int)(lat % ncl);
-    const int d = (int)(lat / ncl);
-    const int64_t Nf = 10LL * a.nf * a.nf * a.nrf;
-    const int bmode = rc == 0 ? a.bc_cmb : TN_NEUMANN;
-    const int tmode = rc == ncr - 1 ? a.bc_surf : TN_NEUMANN;
-    const int frw = rc & 1;
-    for (int w = 0; w < 2; ++w) {
-        WedgeGeo G;
-        load_wedge(a.coords, a.radii, a.nf, d, x, y, rc, w, G);
-        double pc[6];
-        for (int i = 0; i < 6; ++i) {
-            const int64_t ic = (((int64_t)d * a.nc + x / 2 + WOFF[w][i][0]) * a.nc + y / 2 + WOFF[w][i][1]) * a.nrc +
-                               rc / 2 + WOFF[w][i][2];
-            pc[i] = (double)a.p[ic];
-        }
-        const int flw = fine_lateral_wedge_idx(x, y, w);
-        double out[6][3] = {{0}};
-        for (int q = 0; q < 6; ++q) {
-            double g[6][3];
-            const double adet = wedge_qp(G, Q6P[q], g);
-            double ph = 0.0;
-            for (int i = 0; i < 6; ++i) ph += shape_coarse(i, frw, flw, Q6P[q]) * pc[i];
-            const double f = -Q6W * adet * ph;
-            for (int j = 0; j < 6; ++j)
-                for (int c = 0; c < 3; ++c) out[j][c] += f * g[j][c];
-        }
-        for (int j = 0; j < 6; ++j) {
-            const int mode = j < 3 ? bmode : tmode;
-            if (mode == TN_DIRICHLET) continue;
-            const int64_t lj = ((int64_t)d * a.nf + x + WOFF[w][j][0]) * a.nf + y + WOFF[w][j][1];
-            if (mode == TN_FREESLIP) {
-                double nn[3];
-                unit_normal(a.coords, lj, nn);
-                const double on = nn[0] * out[j][0] + nn[1] * out[j][1] + nn[2] * out[j][2];
-                for (int c = 0; c < 3; ++c) out[j][c] -= on * nn[c];
+    T al[2], sgm[2], bx[2], be[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const V3<T> du1{u1[t].x - u0[t].x, u1[t].y - u0[t].y, u1[t].z - u0[t].z};
+        const V3<T> du2{u2[t].x - u0[t].x, u2[t].y - u0[t].y, u2[t].z - u0[t].z};
+        al[t] = dot3(L.v1, du1) + dot3(L.v3, du2);
+        sgm[t] = dot3(L.nv, u0[t]);
+        bx[t] = dot3(L.nv, du1);
+        be[t] = dot3(L.nv, du2);
+    }
+#pragma unroll
+    for (int ir = 0; ir < 2; ++ir) {
+        const T ca0 = rm.C[ir][0] - rm.A[ir][0], ca1 = rm.C[ir][1] - rm.A[ir][1];
+        const T P0 = L.sg * (al[0] * rm.A[ir][0] + sgm[0] * rm.C[ir][0] + al[1] * rm.A[ir][1] + sgm[1] * rm.C[ir][1]);
+        const T Px = L.sg * (bx[0] * ca0 + bx[1] * ca1);
+        const T Pe = L.sg * (be[0] * ca0 + be[1] * ca1);
+#pragma unroll
+        for (int il = 0; il < 3; ++il) out[il][ir] += P0 * M[il][0] + Px * M[il][1] + Pe * M[il][2];
+    }
+}
+
+// (B^T p) contributions of one fine wedge to its six velocity nodes:
+// p[il][ir] coarse values, y0/y1/y2 at node levels t = 0, 1 (accumulated)
+template <typename T>
+__device__ inline void grad_wedge(const LatWedge<T>& L, const T (*M)[3], const RadMoments<T>& rm, const T p[3][2],
+                                  T y0[2][3], T y1[2][3], T y2[2][3])
+{
+    T q0[2], qx[2], qe[2];
+#pragma unroll
+    for (int ir = 0; ir < 2; ++ir) {
+        q0[ir] = L.sg * (p[0][ir] * M[0][0] + p[1][ir] * M[1][0] + p[2][ir] * M[2][0]);
+        qx[ir] = L.sg * (p[0][ir] * M[0][1] + p[1][ir] * M[1][1] + p[2][ir] * M[2][1]);
+        qe[ir] = L.sg * (p[0][ir] * M[0][2] + p[1][ir] * M[1][2] + p[2][ir] * M[2][2]);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const T X = rm.A[0][t] * q0[0] + rm.A[1][t] * q0[1];
+        const T Y = rm.C[0][t] * q0[0] + rm.C[1][t] * q0[1];
+        const T ca0 = rm.C[0][t] - rm.A[0][t], ca1 = rm.C[1][t] - rm.A[1][t];
+        const T Zx = ca0 * qx[0] + ca1 * qx[1];
+        const T Ze = ca0 * qe[0] + ca1 * qe[1];
+        const T s0 = Y - Zx - Ze;
+        y0[t][0] += s0 * L.nv.x - X * (L.v1.x + L.v3.x);
+        y0[t][1] += s0 * L.nv.y - X * (L.v1.y + L.v3.y);
+        y0[t][2] += s0 * L.nv.z - X * (L.v1.z + L.v3.z);
+        y1[t][0] += X * L.v1.x + Zx * L.nv.x;
+        y1[t][1] += X * L.v1.y + Zx * L.nv.y;
+        y1[t][2] += X * L.v1.z + Zx * L.nv.z;
+        y2[t][0] += X * L.v3.x + Ze * L.nv.x;
+        y2[t][1] += X * L.v3.y + Ze * L.nv.y;
+        y2[t][2] += X * L.v3.z + Ze * L.nv.z;
+    }
+}
+
+// Lateral corners of a quad: k = 0 (x, y), 1 (x+1, y), 2 (x, y+1), 3 (x+1, y+1);
+// wedge w's vertices 0, 1, 2 in that numbering (WOFF[w][0..2]).
+__device__ constexpr int WK[2][3] = {{0, 1, 2}, {3, 2, 1}};
+
+// B: a wave per coarse lateral quad (X, Y) and chunk of 64 coarse radial
+// cells (one per lane).  A lane sums the 16 fine wedges of its coarse cell
+// into the cell's 8 corners; the shared radial level is merged by a lane
+// shuffle, so a lane adds its lower 4 corners to p with one atomic each and
+// the chunk's top lane the upper 4.
+template <typename T>
+__global__ __launch_bounds__(256) void k_div(DivArgs<T> a)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nqc = a.nc - 1, ncrc = a.nrc - 1;
+    const int nch = (ncrc + 63) >> 6;
+    if (wave >= 10LL * nqc * nqc * nch) return;  // wave-uniform
+    int64_t t = wave;
+    const int h = (int)(t % nch);
+    t /= nch;
+    const int Y = (int)(t % nqc);
+    t /= nqc;
+    const int X = (int)(t % nqc);
+    const int d = (int)(t / nqc);
+    const int R = 64 * h + lane;
+    const bool valid = R < ncrc;
+    const int Rc = valid ? R : ncrc - 1;
+    const int toplane = min(63, ncrc - 1 - 64 * h);
+    const int nf = a.nf, nrf = a.nrf;
+    const int64_t Nf = 10LL * nf * nf * nrf;
+    const double sc = valid ? 1.0 : 0.0;
+    const RadMoments<T> rm[2] = {radial_moments<T>(a.radii[2 * Rc], a.radii[2 * Rc + 1], 0, sc),
+                                 radial_moments<T>(a.radii[2 * Rc + 1], a.radii[2 * Rc + 2], 1, sc)};
+    const int bl = Rc == 0 ? a.bc_cmb : TN_NEUMANN;           // fine level 2R
+    const int tl = Rc == ncrc - 1 ? a.bc_surf : TN_NEUMANN;  // fine level 2R + 2
+    T acc[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    // one fine quad at a time (all four at once would exceed the register file in FP64)
+#pragma unroll 1
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll 1
+        for (int dx = 0; dx < 2; ++dx) {
+            const int fx = 2 * X + dx, fy = 2 * Y + dy;
+            const int64_t l00 = ((int64_t)d * nf + fx) * nf + fy;
+            const int64_t latc[4] = {l00, l00 + nf, l00 + 1, l00 + nf + 1};
+            V3<T> u[4][3];
+            V3<T> P[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int64_t ib = latc[k] * nrf + 2 * Rc;
+#pragma unroll
+                for (int l = 0; l < 3; ++l) u[k][l] = {a.u[ib + l], a.u[Nf + ib + l], a.u[2 * Nf + ib + l]};
+                P[k] = load_coord<T>(a.coords, latc[k]);
             }
-            const int64_t ij = lj * a.nrf + rc + WOFF[w][j][2];
-            for (int c = 0; c < 3; ++c) unsafeAtomicAdd(a.u + c * Nf + ij, (T)out[j][c]);
+            if (bl != TN_NEUMANN || tl != TN_NEUMANN) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (bl != TN_NEUMANN) trial_bc(u[k][0], bl, P[k]);
+                    if (tl != TN_NEUMANN) trial_bc(u[k][2], tl, P[k]);
+                }
+            }
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                const int k0 = WK[w][0], k1 = WK[w][1], k2 = WK[w][2];
+                const LatWedge<T> L = lat_wedge(P[k0], P[k1], P[k2]);
+                const int flw = fine_lateral_wedge_idx(dx, dy, w);
+                T M[3][3];
+#pragma unroll
+                for (int il = 0; il < 3; ++il)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) M[il][j] = (T)LATM.m[flw][il][j];
+                T out[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+                for (int frw = 0; frw < 2; ++frw) {
+                    const V3<T> u0[2] = {u[k0][frw], u[k0][frw + 1]};
+                    const V3<T> u1[2] = {u[k1][frw], u[k1][frw + 1]};
+                    const V3<T> u2[2] = {u[k2][frw], u[k2][frw + 1]};
+                    div_wedge(L, M, rm[frw], u0, u1, u2, out);
+                }
+                // coarse corners of wedge w: WOFF[w][il] in the k numbering
+#pragma unroll
+                for (int il = 0; il < 3; ++il) {
+                    acc[WK[w][il]][0] += out[il][0];
+                    acc[WK[w][il]][1] += out[il][1];
+                }
+            }
+        }
+    const int nc = a.nc, nrc = a.nrc;
+    const int64_t c00 = ((int64_t)d * nc + X) * nc + Y;
+    const int64_t latc[4] = {c00, c00 + nc, c00 + 1, c00 + nc + 1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        T below = __shfl_up(acc[k][1], 1, 64);
+        if (lane == 0) below = T(0);
+        if (valid) unsafeAtomicAdd(a.p + latc[k] * nrc + R, acc[k][0] + below);
+        if (lane == toplane) unsafeAtomicAdd(a.p + latc[k] * nrc + R + 1, acc[k][1]);
+    }
+}
+
+// B^T: the atomic-free lateral march of the epsilon kernel over fine cells
+// (a block of W waves owns W-1 node rows; wave w marches fine quad strip
+// x0 - 1 + w in y; lanes are fine radial cells, chunks of 64): each fine cell
+// evaluates its two wedges from the 8 coarse values of its coarse cell and
+// routes the 8 corner sums -- (x, y) complete, (x, y+1) and (x+1, y+1)
+// carried, (x+1, y) through LDS to the next wave, radial levels by a lane
+// shuffle.  Every velocity node copy is written once (store or
+// read-modify-write); Dirichlet shells are left out and free-slip shells
+// projected tangentially on the summed value (the test-side masks and
+// rotations of gradient.hpp:115-325).
+template <typename T, int W>
+__global__ __launch_bounds__(64 * W) void k_grad_march(GradArgs<T> a)
+{
+    constexpr int OWN = W - 1;
+    __shared__ T ebuf[2][W][3][65];
+    __shared__ T lvl[W][MARCH_SEG_MAX][3];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int n = a.nf, nr = a.nrf, ncr = nr - 1;
+    const int nc = a.nc, nrc = a.nrc;
+    const int nxb = (n + OWN - 1) / OWN, nys = (n + a.seg - 1) / a.seg;
+    int b = blockIdx.x;
+    const int ys = b % nys;
+    b /= nys;
+    const int xb = b % nxb;
+    const int d = b / nxb;
+    const int x = xb * OWN - 1 + w;
+    const bool strip_ok = x >= 0 && x <= n - 2;
+    const bool row_own = w >= 1 && x <= n - 1;
+    const int y0 = ys * a.seg, y1 = min(y0 + a.seg, n);
+    const int64_t N = 10LL * n * n * nr;
+    const int nchunks = (ncr + 63) >> 6;
+    int step = 0;  // LDS double-buffer parity, continuous over chunks
+
+    for (int h = 0; h < nchunks; ++h) {
+        const int c = 64 * h + lane;
+        const bool valid = c < ncr;
+        const int toplane = min(63, ncr - 1 - 64 * h);
+        const bool last = h == nchunks - 1;
+        const int cc = valid ? c : ncr - 1;
+        const RadMoments<T> rm = radial_moments<T>(a.radii[cc], a.radii[cc + 1], cc & 1, valid ? 1.0 : 0.0);
+        const int R = cc >> 1;
+        T c01[3] = {0, 0, 0}, c11[3] = {0, 0, 0}, c01t[3] = {0, 0, 0}, c11t[3] = {0, 0, 0};
+
+        for (int qy = y0 - 1; qy <= y1 - 1; ++qy, ++step) {
+            const bool quad_ok = strip_ok && qy >= 0 && qy <= n - 2;
+            T(*E)[65] = ebuf[step & 1][w];
+            const int64_t lat00 = ((int64_t)d * n + x) * n + qy;
+            T yb[4][3], yt[4][3];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
+            if (quad_ok) {
+                const int64_t c00 = ((int64_t)d * nc + (x >> 1)) * nc + (qy >> 1);
+                const int64_t cl[4] = {c00, c00 + nc, c00 + 1, c00 + nc + 1};
+                T pc[4][2];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    pc[k][0] = a.p[cl[k] * nrc + R];
+                    pc[k][1] = a.p[cl[k] * nrc + R + 1];
+                }
+                const int64_t latc[4] = {lat00, lat00 + n, lat00 + 1, lat00 + n + 1};
+                V3<T> P[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) P[k] = load_coord<T>(a.coords, latc[k]);
+#pragma unroll
+                for (int w2 = 0; w2 < 2; ++w2) {
+                    const int k0 = WK[w2][0], k1 = WK[w2][1], k2 = WK[w2][2];
+                    const LatWedge<T> L = lat_wedge(P[k0], P[k1], P[k2]);
+                    const int flw = fine_lateral_wedge_idx(x & 1, qy & 1, w2);
+                    T M[3][3];
+#pragma unroll
+                    for (int il = 0; il < 3; ++il)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) M[il][j] = (T)LATM.m[flw][il][j];
+                    const T p[3][2] = {{pc[k0][0], pc[k0][1]}, {pc[k1][0], pc[k1][1]}, {pc[k2][0], pc[k2][1]}};
+                    T o0[2][3] = {{0, 0, 0}, {0, 0, 0}}, o1[2][3] = {{0, 0, 0}, {0, 0, 0}},
+                      o2[2][3] = {{0, 0, 0}, {0, 0, 0}};
+                    grad_wedge(L, M, rm, p, o0, o1, o2);
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        yb[k0][j] += o0[0][j];
+                        yt[k0][j] += o0[1][j];
+                        yb[k1][j] += o1[0][j];
+                        yt[k1][j] += o1[1][j];
+                        yb[k2][j] += o2[0][j];
+                        yt[k2][j] += o2[1][j];
+                    }
+                }
+            }
+            // radial merge (node level c = bottom of cell c + top of cell c-1) and routing
+            T part[3], tp[3] = {0, 0, 0};
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                T Mg[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    T up = __shfl_up(yt[k][j], 1, 64);
+                    if (lane == 0) up = T(0);
+                    Mg[k] = yb[k][j] + up;
+                }
+                part[j] = Mg[0] + c01[j];
+                c01[j] = Mg[2];
+                E[j][lane] = Mg[1] + c11[j];
+                c11[j] = Mg[3];
+            }
+            if (lane == toplane) {  // level above this chunk's last cell
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    tp[j] = yt[0][j] + c01t[j];
+                    c01t[j] = yt[2][j];
+                    E[j][64] = yt[1][j] + c11t[j];
+                    c11t[j] = yt[3][j];
+                }
+            }
+            __syncthreads();
+            if (row_own && qy >= y0) {
+                const T(*Ein)[65] = ebuf[step & 1][w - 1];
+                const int64_t lat = lat00;
+                const int slot = qy - y0;
+                auto finish = [&](V3<T> v, int level, int mode) {
+                    const int64_t idx = lat * nr + level;
+                    if (mode == TN_DIRICHLET) {
+                        if (a.add) return;
+                        v = {T(0), T(0), T(0)};
+                    } else if (mode == TN_FREESLIP) {
+                        const V3<T> nn = unit_of(load_coord<T>(a.coords, lat));
+                        const T vn = dot3(nn, v);
+                        v = {v.x - vn * nn.x, v.y - vn * nn.y, v.z - vn * nn.z};
+                    }
+                    if (a.add) {
+                        a.u[idx] += v.x;
+                        a.u[N + idx] += v.y;
+                        a.u[2 * N + idx] += v.z;
+                    } else {
+                        a.u[idx] = v.x;
+                        a.u[N + idx] = v.y;
+                        a.u[2 * N + idx] = v.z;
+                    }
+                };
+                if (valid) {
+                    V3<T> v{part[0] + Ein[0][lane], part[1] + Ein[1][lane], part[2] + Ein[2][lane]};
+                    if (lane == 0 && h > 0) {
+                        v.x += lvl[w][slot][0];
+                        v.y += lvl[w][slot][1];
+                        v.z += lvl[w][slot][2];
+                    }
+                    finish(v, c, c == 0 ? a.bc_cmb : TN_NEUMANN);
+                }
+                if (lane == toplane) {
+                    const V3<T> v{tp[0] + Ein[0][64], tp[1] + Ein[1][64], tp[2] + Ein[2][64]};
+                    if (last) {
+                        finish(v, ncr, a.bc_surf);
+                    } else {
+                        lvl[w][slot][0] = v.x;
+                        lvl[w][slot][1] = v.y;
+                        lvl[w][slot][2] = v.z;
+                    }
+                }
+            }
         }
     }
 }
@@ -303,16 +616,22 @@ __global__ __launch_bounds__(128) void k_mass(MassArgs<T> a)
 template <typename T>
 void launch_div(const DivArgs<T>& a, hipStream_t s)
 {
-    const int64_t ncell = 10LL * (a.nf - 1) * (a.nf - 1) * (a.nrf - 1);
-    hipLaunchKernelGGL(k_div<T>, dim3((unsigned)((ncell + 127) / 128)), dim3(128), 0, s, a);
+    TN_REQUIRE(a.nf == 2 * a.nc - 1 && a.nrf == 2 * a.nrc - 1 && a.nrc >= 2 && a.nc >= 2,
+               "B: velocity mesh must be the pressure mesh refined once");
+    const int64_t waves = 10LL * (a.nc - 1) * (a.nc - 1) * ((a.nrc - 1 + 63) / 64);
+    hipLaunchKernelGGL(k_div<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a);
     TN_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_grad(const GradArgs<T>& a, hipStream_t s)
 {
-    const int64_t ncell = 10LL * (a.nf - 1) * (a.nf - 1) * (a.nrf - 1);
-    hipLaunchKernelGGL(k_grad<T>, dim3((unsigned)((ncell + 127) / 128)), dim3(128), 0, s, a);
+    TN_REQUIRE(a.nf == 2 * a.nc - 1 && a.nrf == 2 * a.nrc - 1 && a.nrc >= 2 && a.nc >= 2,
+               "B^T: velocity mesh must be the pressure mesh refined once");
+    TN_REQUIRE(a.seg >= 1 && a.seg <= MARCH_SEG_MAX, "B^T: segment length out of range");
+    constexpr int W = GRAD_MARCH_WAVES;
+    const int nxb = (a.nf + W - 2) / (W - 1), nys = (a.nf + a.seg - 1) / a.seg;
+    hipLaunchKernelGGL((k_grad_march<T, W>), dim3((unsigned)(10 * nxb * nys)), dim3(64 * W), 0, s, a);
     TN_HIP(hipGetLastError());
 }
 

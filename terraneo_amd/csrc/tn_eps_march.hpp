@@ -30,6 +30,8 @@ struct MarchArgs {
 int march_block_rows(int dtype_size);
 // resident waves per CU at the kernel's register use (element size in bytes)
 int march_waves_per_cu(int dtype_size);
+// segment length of a march kernel (own node rows per block, resident waves per CU)
+int march_seg_for(int n, int own, int waves_per_cu);
 
 // boundary-node quadratic forms e^T B e (see k_boundary_blocks)
 template <typename T>

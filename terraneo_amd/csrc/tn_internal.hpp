@@ -130,6 +130,9 @@ void launch_lincomb(int64_t count, T* y, int nx, const double* c, const T* const
 // out_dev[0..nd) += sum over owned nodes, accumulate in double; ncomp fields of N each
 template <typename T>
 void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, double* out_dev, hipStream_t s);
+// y[i] -= scale * sum_dev[0], i < count
+template <typename T>
+void launch_sub_scaled(T* y, int64_t count, const double* sum_dev, double scale, hipStream_t s);
 
 // fs/fs penalty: beta[0..3) = sum_owned r x u  ;  y (+)= gamma x r with gamma = S beta
 template <typename T>

@@ -22,8 +22,13 @@ struct GradArgs {
     int nf, nrf, nc, nrc;
     int bc_cmb, bc_surf;
     const T* p;  // coarse pressure
-    T* u;        // fine velocity (accumulated)
+    T* u;        // fine velocity
+    int seg;     // node columns per block of the march
+    int add;     // 1: u += B^T p, 0: u = B^T p
 };
+
+// waves per block of the B^T march (node rows per block + 1)
+constexpr int GRAD_MARCH_WAVES = 8;
 
 template <typename T>
 struct MassArgs {

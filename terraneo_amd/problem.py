@@ -31,7 +31,7 @@ class StokesProblem:
     levels (the coarsest has lateral level lat-levels+1); bc: (CMB, surface)."""
 
     def __init__(self, lat, rad, levels, r_min=0.55, r_max=1.0, bc=(TN_FREESLIP, TN_FREESLIP), device="cuda:0",
-                 prec_dtype="f64", coarsest_f64=True, gmg=None, gca_bracket="reference"):
+                 prec_dtype="f64", coarsest_f64=True, gmg=None, gca_bracket="reference", coarse_ops="gca"):
         torch = _torch()
         assert levels >= 2 and lat - levels + 1 >= 0 and rad - levels + 1 >= 1
         self.device = torch.device(device)
@@ -43,6 +43,10 @@ class StokesProblem:
         self.prec_dtype = prec_dtype
         self.coarsest_f64 = coarsest_f64
         self.gca_bracket = gca_bracket
+        assert coarse_ops in ("gca", "rediscretize")
+        # "gca": Galerkin matrices (gca.hpp); "rediscretize": matrix-free coarse
+        # operators with the nodal viscosity injected (the assess tests' setup)
+        self.coarse_ops = coarse_ops
         self.gmg_opts = gmg or api.gmg_options(cheby_order=3, cheby_iterations=1, power_iterations=10, num_cycles=1)
         self.f64 = dict(dtype=torch.float64, device=self.device)
         self.A = None
@@ -77,14 +81,23 @@ class StokesProblem:
             fine = api.Eps(self.vel, self._eta_p, *bc, eta_cell=self._eta_cell_p)
         ops = [fine]
         self._dummies = []
+        if self.coarse_ops == "rediscretize":
+            assert self.eta is not None, "rediscretized coarse operators need a nodal viscosity"
+            eta_l = self.eta.view(10, self.vel.n, self.vel.n, self.vel.nr)
         for k, m in enumerate(reversed(self.meshes[:-1])):
             coarsest = k == len(self.meshes) - 2
             dt = torch.float64 if (coarsest and self.coarsest_f64) else pd
             sdt = TN_F64 if dt == torch.float64 else st
-            one = torch.ones(m.N, dtype=dt, device=self.device)  # placeholder viscosity, unused once stored
-            self._dummies.append(one)
-            op = api.Eps(m, one, *bc)
-            op.gca_from(ops[-1], sdt, bracket=self.gca_bracket)
+            if self.coarse_ops == "rediscretize":
+                eta_l = eta_l[:, ::2, ::2, ::2]
+                e = eta_l.contiguous().view(-1).to(dt)
+                self._dummies.append(e)
+                op = api.Eps(m, e, *bc)
+            else:
+                one = torch.ones(m.N, dtype=dt, device=self.device)  # placeholder viscosity, unused once stored
+                self._dummies.append(one)
+                op = api.Eps(m, one, *bc)
+                op.gca_from(ops[-1], sdt, bracket=self.gca_bracket)
             ops.append(op)
         self.ops = list(reversed(ops))
         self.gmg = api.Gmg(self.ops, self.gmg_opts)
@@ -129,13 +142,16 @@ class StokesProblem:
         return bu, bp
 
     # ---------------------------------------------------------------- solve
-    def solve(self, bu, bp, method="fgmres", rtol=1e-8, max_iterations=500, restart=30, u=None, p=None):
+    def solve(self, bu, bp, method="fgmres", rtol=1e-8, max_iterations=500, restart=30, u=None, p=None,
+              pressure_mean_free=False):
+        """pressure_mean_free: solve with the pressure constrained to zero mean
+        (see tn_krylov_options.pressure_mean_free)."""
         torch = _torch()
         u = torch.zeros(3 * self.vel.N, **self.f64) if u is None else u
         p = torch.zeros(self.pre.N, **self.f64) if p is None else p
         meth = TN_FGMRES_BLOCKTRI if method == "fgmres" else TN_MINRES_BLOCKDIAG
         info = api.stokes_solve(self.K, meth, self.gmg, self.schur, u, p, bu, bp, max_iterations=max_iterations,
-                                rtol=rtol, atol=1e-30, restart=restart)
+                                rtol=rtol, atol=1e-30, restart=restart, pressure_mean_free=pressure_mean_free)
         return u, p, info
 
     def residual(self, u, p, bu, bp):
