@@ -59,6 +59,12 @@ typedef struct tn_gmg_options {
     double coarse_atol;
     uint64_t seed;             /* power-iteration start vector               */
     int coarse_solver;         /* TN_COARSE_PCG (the reference's) or TN_COARSE_DENSE */
+    /* 0: the reference's V-cycle.  1: on levels with free-slip shells the
+     * fine residual is projected tangentially before restriction and the
+     * iterate after the prolongated correction (P' = Pi P, R' = P^T Pi with
+     * Pi the free-slip projection): the constant prolongation of a coarse
+     * tangential field is not tangential at the fine boundary nodes. */
+    int freeslip_projected_transfer;
 } tn_gmg_options;
 
 /* coarsest-level solver of the multigrid: unpreconditioned PCG as the

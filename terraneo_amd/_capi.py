@@ -20,7 +20,7 @@ TN_GCA_REFERENCE, TN_GCA_GEOMETRIC = 0, 1
 class GmgOptions(C.Structure):
     _fields_ = [("cheby_order", C.c_int), ("cheby_iterations", C.c_int), ("power_iterations", C.c_int),
                 ("num_cycles", C.c_int), ("coarse_max_iterations", C.c_int), ("coarse_rtol", C.c_double),
-                ("coarse_atol", C.c_double), ("seed", C.c_uint64), ("coarse_solver", C.c_int)]
+                ("coarse_atol", C.c_double), ("seed", C.c_uint64), ("coarse_solver", C.c_int), ("freeslip_projected_transfer", C.c_int)]
 
 
 class KrylovOptions(C.Structure):

@@ -234,10 +234,11 @@ class Stokes:
 
 
 def gmg_options(cheby_order=2, cheby_iterations=3, power_iterations=10, num_cycles=1, coarse_max_iterations=1000,
-                coarse_rtol=1e-6, coarse_atol=1e-16, seed=12345, coarse_solver="pcg"):
+                coarse_rtol=1e-6, coarse_atol=1e-16, seed=12345, coarse_solver="pcg",
+                freeslip_projected_transfer=False):
     cs = {"pcg": 0, "dense": 1}[coarse_solver]
     return GmgOptions(cheby_order, cheby_iterations, power_iterations, num_cycles, coarse_max_iterations, coarse_rtol,
-                      coarse_atol, seed, cs)
+                      coarse_atol, seed, cs, int(bool(freeslip_projected_transfer)))
 
 
 class Gmg:

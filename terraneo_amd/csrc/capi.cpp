@@ -163,17 +163,20 @@ void eps_apply_t(tn_eps* op, const void* src_v, void* dst_v, hipStream_t s)
                            m.n, m.nr, march_seg(m.n, (int)sizeof(T)), op->bc_cmb, op->bc_surf, add ? 1 : 0};
         tn::launch_eps_march<T>(a, op->diagonal, s);
     } else {
-        if (penalty)
-            tn::launch_penalty_init<T>(m, op->pen.p, op->pen.p + 4, dst, add, s);
-        else if (!add)
-            tn::launch_fill<T>(dst, 3 * m.N, T(0), s);
+        // the gather writes every node copy once: the penalty term first, then accumulate
+        if (penalty) tn::launch_penalty_init<T>(m, op->pen.p, op->pen.p + 4, dst, add, s);
+        const int acc = (add || penalty) ? 1 : 0;
+        const int64_t ncs = tn::stored_cellsum_elems(m.n, m.nr);
+        if (op->cellsum.n != (size_t)ncs * sizeof(T)) op->cellsum.alloc((size_t)ncs * sizeof(T));
+        T* csum = (T*)op->cellsum.p;
+        T* ctop = csum + 12 * (10LL * (m.n - 1) * (m.n - 1) * (m.nr - 1));
         if (op->stored_dtype == TN_F64) {
             tn::StoredArgs<T, double> a{(const double*)op->stored.p, src, dst, m.d_coords.p, m.n, m.nr, op->bc_cmb,
-                                        op->bc_surf};
+                                        op->bc_surf, acc, csum, ctop};
             tn::launch_stored_apply<T, double>(a, op->diagonal, s);
         } else {
             tn::StoredArgs<T, float> a{(const float*)op->stored.p, src, dst, m.d_coords.p, m.n, m.nr, op->bc_cmb,
-                                       op->bc_surf};
+                                       op->bc_surf, acc, csum, ctop};
             tn::launch_stored_apply<T, float>(a, op->diagonal, s);
         }
     }
@@ -480,7 +483,7 @@ int tn_eps_gca_from_bracket(tn_eps* coarse, const tn_eps* fine, int store_dtype,
     auto& mf = const_cast<tn_mesh*>(fine->mesh)->lvl.dev();
     TN_REQUIRE(2 * (mc.n - 1) == mf.n - 1 && 2 * (mc.nr - 1) == mf.nr - 1,
                "GCA: fine mesh must be the uniform refinement of the coarse mesh");
-    const int64_t bytes = coarse->cells() * 2 * tn::PACKED * (store_dtype == TN_F64 ? 8 : 4);
+    const int64_t bytes = tn::stored_elems(coarse->cells()) * (store_dtype == TN_F64 ? 8 : 4);
     coarse->stored.alloc((size_t)bytes);
     coarse->stored_dtype = store_dtype;
     const bool fine_geo = fine->stored_dtype < 0;
@@ -521,11 +524,11 @@ int tn_eps_get_local_matrix(const tn_eps* op, int sd, int x, int y, int r, int w
                    (wedge == 0 || wedge == 1),
                "local matrix index out of range");
     const int64_t cell = (((int64_t)sd * (m.n - 1) + x) * (m.n - 1) + y) * (m.nr - 1) + r;
-    const int64_t nc = op->cells();
     const size_t es = op->stored_dtype == TN_F64 ? 8 : 4;
     std::vector<double> packed(tn::PACKED);
     for (int k = 0; k < tn::PACKED; ++k) {
-        const size_t off = ((size_t)(wedge * tn::PACKED + k) * nc + cell) * es;
+        const size_t off =
+            (size_t)(es == 8 ? tn::stored_at<double>(cell, wedge, k) : tn::stored_at<float>(cell, wedge, k)) * es;
         if (es == 8) {
             TN_HIP(hipMemcpy(&packed[k], op->stored.p + off, 8, hipMemcpyDeviceToHost));
         } else {

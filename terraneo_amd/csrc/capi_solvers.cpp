@@ -197,8 +197,14 @@ void gmg_setup(tn_gmg* g, const tn_gmg_options& o, hipStream_t s)
     for (int l = 0; l < L; ++l) {
         lo.push_back(g->lops[l].get());
         sms.push_back(g->sm[l].get());
-        if (l < L - 1)
-            tr.push_back(Transfer{&const_cast<tn_mesh*>(g->ops[l]->mesh)->lvl, &const_cast<tn_mesh*>(g->ops[l + 1]->mesh)->lvl});
+        if (l < L - 1) {
+            Transfer t{&const_cast<tn_mesh*>(g->ops[l]->mesh)->lvl, &const_cast<tn_mesh*>(g->ops[l + 1]->mesh)->lvl};
+            if (o.freeslip_projected_transfer) {
+                t.project_cmb = g->ops[l + 1]->bc_cmb == TN_FREESLIP;
+                t.project_surf = g->ops[l + 1]->bc_surf == TN_FREESLIP;
+            }
+            tr.push_back(t);
+        }
     }
     g->mg.reset(new Multigrid(lo, tr, sms, coarse, g->tmp, g->r, g->e, o.num_cycles));
 }

@@ -11,9 +11,10 @@
 // Storage of coarse operators: the two 18x18 wedge matrices of every hex cell
 // are symmetric (P^T A P of a symmetric A), so only the upper triangle is
 // kept: 171 entries per wedge, dimension-wise dof order (d*6 + node), packed
-// row-major.  Entries are stored entry-major, [2*171][cells] with cells in
-// (diamond, x, y, r) order, r fastest: neighbouring lanes (radially adjacent
-// cells) read neighbouring addresses, so the matrix stream is coalesced.
+// row-major.  Cells (diamond, x, y, r order, r fastest) are grouped in tiles
+// of 64, interleaved in 16-byte vectors inside a tile (stored_at in tn_mg.hpp): neighbouring
+// lanes (radially adjacent cells) read neighbouring addresses and one wave's
+// quad reads one contiguous block.
 // The boundary masks and free-slip rotations are applied at apply time, as
 // the reference does for its stored matrices.
 #include "tn_internal.hpp"
@@ -25,6 +26,33 @@ namespace {
 __device__ constexpr int WOFF[2][6][3] = {{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}},
                                          {{1, 1, 0}, {0, 1, 0}, {1, 0, 0}, {1, 1, 1}, {0, 1, 1}, {1, 0, 1}}};
 
+struct V3d {
+    double x, y, z;
+};
+
+// 16-byte vectors of stored entries
+template <typename S>
+struct VecSel;
+template <>
+struct VecSel<float> {
+    using type = float4;
+};
+template <>
+struct VecSel<double> {
+    using type = double2;
+};
+template <typename S>
+using VecOf = typename VecSel<S>::type;
+__device__ __forceinline__ float vec_get(const float4& v, int i)
+{
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ double vec_get(const double2& v, int i) { return i == 0 ? v.x : v.y; }
+
+// A wave-uniform pointer, made opaque through readfirstlane so that a load
+// p[off32] becomes one global_load with an SGPR base and a 32-bit VGPR offset
+// (otherwise the address arithmetic is folded into a 64-bit VGPR address per
+// load, which for a 171-entry matrix row stream costs ~340 VGPRs of addresses).
 __host__ __device__ constexpr int pk(int i, int j)  // packed upper index, i <= j
 {
     return i * 18 - (i * (i - 1)) / 2 + (j - i);
@@ -198,7 +226,7 @@ __device__ inline void stored_block(const S* __restrict__ st, int64_t ncells, in
                 r = c;
                 c = t;
             }
-            A[i][j] = (double)st[(int64_t)(w * 171 + pk(r, c)) * ncells + cell];
+            A[i][j] = (double)st[stored_at<S>(cell, w, pk(r, c))];
         }
 }
 
@@ -270,62 +298,207 @@ __global__ void k_gca_build(GcaArgs<T, S, SF> a)
     for (int i = 0; i < 6; ++i)
         for (int j = 0; j < 6; ++j) {
             const int r = i + 6 * q, c = j + 6 * p;
-            if (r <= c) a.stored_c[(int64_t)(W * 171 + pk(r, c)) * ncc + cell] = (S)out[i][j];
+            if (r <= c) a.stored_c[stored_at<S>(cell, W, pk(r, c))] = (S)out[i][j];
         }
 }
 
 // ---------------------------------------------------------------- stored-matrix apply
+//
+// Three launches, every node copy finished once, deterministic:
+//  * k_stored_cells: a wave per lateral quad and chunk of 64 radial cells
+//    (one per lane).  Each cell not in a Dirichlet / free-slip boundary layer
+//    applies its two wedge matrices (plain product, or their diagonal) read
+//    as coalesced 16-byte vectors of the tile layout (stored_at); the cell's
+//    8 corner sums are merged radially with a lane shuffle (node level c =
+//    bottom of cell c + top of cell c-1) and the 4 lateral corners x 3
+//    components at level c are written per cell, the chunk's top level once
+//    per chunk.  No barriers, no atomics: the matrix stream runs at the
+//    occupancy the registers allow.
+//  * k_stored_gather: a thread per node copy sums its (at most 4) lateral
+//    quads' corner values and stores (Replace) or adds (Add) the result.
+//  * k_stored_boundary: the (at most two) boundary cell layers with the
+//    boundary treatment the reference applies to its stored matrices
+//    (epsilon_divdiv_kerngen.hpp:823-1051) -- trial side masked (Dirichlet)
+//    or projected (free-slip), test side replaced by the Dirichlet diagonal or
+//    completed by (n^T A_bb n)(n . x) n; diagonal mode keeps diag(A) and, at
+//    free-slip nodes, R^T diag(R A_bb R^T) R -- added with atomics after the
+//    gather (2 of the nr-1 layers; that code stays out of the streaming
+//    kernel's register budget).
+template <typename T, typename S, bool DIAG>
+__device__ __forceinline__ void stored_cell_sums(const StoredArgs<T, S>& a, int64_t cell, const int64_t latc[4],
+                                                 int cc, T scale, T yb[4][3], T yt[4][3])
+{
+    constexpr int WCORN[2][3] = {{0, 1, 2}, {3, 2, 1}};
+    const int64_t N = 10LL * a.n * a.n * a.nr;
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        T xs[18];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int64_t ib = latc[WCORN[w][j % 3]] * a.nr + cc + (j >= 3);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xs[k * 6 + j] = scale * a.src[k * N + ib];
+        }
+        // dof i of the wedge (comp*6 + node) accumulates into its corner sum
+        auto Y = [&](int i) -> T& {
+            const int node = i % 6, k = i / 6, cr = WCORN[w][node % 3];
+            return node < 3 ? yb[cr][k] : yt[cr][k];
+        };
+        if constexpr (DIAG) {
+#pragma unroll
+            for (int i = 0; i < 18; ++i) Y(i) += (T)a.stored[stored_at<S>(cell, w, pk(i, i))] * xs[i];
+        } else {
+            // packed entries (row-major upper triangle), V per 16-byte vector load
+            constexpr int V = stored_vec<S>();
+            using Vec = VecOf<S>;
+            const Vec* vp = reinterpret_cast<const Vec*>(a.stored + stored_at<S>(cell, w, 0));
+            int k = 0;
+            Vec cur;
+#pragma unroll
+            for (int i = 0; i < 18; ++i) {
+#pragma unroll
+                for (int j = i; j < 18; ++j, ++k) {
+                    if (k % V == 0) cur = vp[(k / V) * 64];
+                    const T aij = (T)vec_get(cur, k % V);
+                    Y(i) += aij * xs[j];
+                    if (j != i) Y(j) += aij * xs[i];
+                }
+            }
+        }
+    }
+}
 
 template <typename T, typename S, bool DIAG>
-__global__ __launch_bounds__(256) void k_stored_apply(StoredArgs<T, S> a)
+__global__ __launch_bounds__(256) void k_stored_cells(StoredArgs<T, S> a)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int ncr = a.nr - 1, ncl = a.n - 1;
+    const int nch = (ncr + 63) >> 6;
+    const int64_t ncol = 10LL * ncl * ncl;
+    if (wave >= ncol * nch) return;  // wave-uniform
+    const int h = (int)(wave % nch);
+    const int64_t col = wave / nch;  // (d, x, y) quad
+    const int y = (int)(col % ncl);
+    const int x = (int)((col / ncl) % ncl);
+    const int d = (int)(col / ((int64_t)ncl * ncl));
+    const int c = 64 * h + lane;
+    const bool valid = c < ncr;
+    const int cc = valid ? c : ncr - 1;
+    const bool boundary_layer = (c == 0 && a.bc_cmb != TN_NEUMANN) || (c == ncr - 1 && a.bc_surf != TN_NEUMANN);
+    const T scale = (valid && !boundary_layer) ? T(1) : T(0);
+    const int64_t ncell = ncol * ncr;
+    const int toplane = min(63, ncr - 1 - 64 * h);
+    const int64_t lat00 = ((int64_t)d * a.n + x) * a.n + y;
+    const int64_t latc[4] = {lat00, lat00 + a.n, lat00 + 1, lat00 + a.n + 1};
+    T yb[4][3], yt[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
+    // (the lane condition also keeps the compiler from hoisting the whole
+    // matrix stream of both wedges into registers ahead of the arithmetic)
+    if (scale != T(0)) stored_cell_sums<T, S, DIAG>(a, col * ncr + cc, latc, cc, scale, yb, yt);
+    // radial merge (level c = bottom of cell c + top of cell c-1) and write-out
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            T up = __shfl_up(yt[k][j], 1, 64);
+            if (lane == 0) up = T(0);
+            if (valid) a.csum[(int64_t)(k * 3 + j) * ncell + col * ncr + c] = yb[k][j] + up;
+            if (lane == toplane) a.ctop[(int64_t)(k * 3 + j) * ncol * nch + col * nch + h] = yt[k][j];
+        }
+}
+
+// Node copy (d, X, Y, level): sum of its lateral quads' corner values.
+template <typename T>
+__global__ __launch_bounds__(256) void k_stored_gather(const T* __restrict__ csum, const T* __restrict__ ctop,
+                                                       T* __restrict__ dst, int n, int nr, int add)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t ncell = 10LL * (a.n - 1) * (a.n - 1) * (a.nr - 1);
-    const bool active = tid < ncell;
-    const int64_t cell = active ? tid : ncell - 1;
+    const int64_t N = 10LL * n * n * nr;
+    if (tid >= N) return;
+    const int ncr = nr - 1, ncl = n - 1;
+    const int nch = (ncr + 63) >> 6;
+    const int64_t ncol = 10LL * ncl * ncl, ncell = ncol * ncr;
+    const int lev = (int)(tid % nr);
+    const int64_t lat = tid / nr;
+    const int Yn = (int)(lat % n);
+    const int Xn = (int)((lat / n) % n);
+    const int d = (int)(lat / ((int64_t)n * n));
+    T v[3] = {T(0), T(0), T(0)};
+    // node = corner k of quad (Xn - (k & 1), Yn - (k >> 1))
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int qx = Xn - (k & 1), qy = Yn - (k >> 1);
+        if (qx < 0 || qy < 0 || qx > ncl - 1 || qy > ncl - 1) continue;
+        const int64_t col = ((int64_t)d * ncl + qx) * ncl + qy;
+        if (lev < ncr) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) v[j] += csum[(int64_t)(k * 3 + j) * ncell + col * ncr + lev];
+        }
+        // first level of a chunk > 0, or the top level: the previous chunk's top
+        if ((lev > 0 && (lev & 63) == 0) || lev == ncr) {
+            const int hp = (lev - 1) >> 6;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) v[j] += ctop[(int64_t)(k * 3 + j) * ncol * nch + col * nch + hp];
+        }
+    }
+    const int64_t idx = tid;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        if (add)
+            dst[j * N + idx] += v[j];
+        else
+            dst[j * N + idx] = v[j];
+    }
+}
+
+// Boundary cell layers: one thread per (layer, lateral cell); atomics into dst.
+template <typename T, typename S, bool DIAG>
+__global__ __launch_bounds__(128) void k_stored_boundary(StoredArgs<T, S> a)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int ncr = a.nr - 1, ncl = a.n - 1;
-    const int rc = (int)(cell % ncr);
-    int64_t lat = cell / ncr;
+    const int64_t nlat = 10LL * ncl * ncl;
+    if (tid >= 2 * nlat) return;
+    const int layer = (int)(tid / nlat);  // 0: CMB cell layer, 1: surface cell layer
+    const int rc = layer == 0 ? 0 : ncr - 1;
+    if ((layer == 0 ? a.bc_cmb : a.bc_surf) == TN_NEUMANN) return;
+    int64_t lat = tid % nlat;
     const int y = (int)(lat % ncl);
     lat /= ncl;
     const int x = (int)(lat % ncl);
     const int d = (int)(lat / ncl);
+    const int64_t ncell = 10LL * ncl * ncl * ncr;
+    const int64_t cell = (((int64_t)d * ncl + x) * ncl + y) * ncr + rc;
     const int64_t N = 10LL * a.n * a.n * a.nr;
     const int64_t lat00 = ((int64_t)d * a.n + x) * a.n + y;
-    // corner lateral ids: 00, 10, 01, 11
     const int64_t latc[4] = {lat00, lat00 + a.n, lat00 + 1, lat00 + a.n + 1};
     constexpr int WCORN[2][3] = {{0, 1, 2}, {3, 2, 1}};
+    // a single-layer shell (ncr == 1) has both shells on this cell
     const int bmode = rc == 0 ? a.bc_cmb : TN_NEUMANN;
     const int tmode = rc == ncr - 1 ? a.bc_surf : TN_NEUMANN;
-    T yb[4][3], yt[4][3];
-    for (int c = 0; c < 4; ++c)
-        for (int k = 0; k < 3; ++k) yb[c][k] = yt[c][k] = T(0);
+    if (layer == 1 && rc == 0) return;  // same cell as layer 0
     double nrm[4][3];
-    if (bmode == TN_FREESLIP || tmode == TN_FREESLIP) {
-        for (int c = 0; c < 4; ++c) {
-            const double* cc = a.coords + 3 * latc[c];
-            const double inv = 1.0 / sqrt(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
-            for (int k = 0; k < 3; ++k) nrm[c][k] = cc[k] * inv;
-        }
+    for (int c = 0; c < 4; ++c) {
+        const double* cp = a.coords + 3 * latc[c];
+        const double inv = 1.0 / sqrt(cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2]);
+        for (int k = 0; k < 3; ++k) nrm[c][k] = cp[k] * inv;
     }
-#pragma unroll
     for (int w = 0; w < 2; ++w) {
         T xs[18];  // dimension-wise: comp*6 + node
-#pragma unroll
         for (int j = 0; j < 6; ++j) {
             const int c = WCORN[w][j % 3];
             const int64_t ib = latc[c] * a.nr + rc + (j >= 3);
-#pragma unroll
             for (int k = 0; k < 3; ++k) xs[k * 6 + j] = a.src[k * N + ib];
         }
-        const S* m = a.stored + (int64_t)(w * 171) * ncell + cell;
-        auto A = [&](int i, int j) -> T { return (T)m[(int64_t)pk(i, j) * ncell]; };
+        auto A = [&](int i, int j) -> T { return (T)a.stored[stored_at<S>(cell, w, pk(i, j))]; };
         T ys[18];
         if constexpr (DIAG) {
-#pragma unroll
             for (int i = 0; i < 18; ++i) ys[i] = A(i, i) * xs[i];
             // free-slip boundary nodes: R^T diag(R A_bb R^T) R x
-#pragma unroll
             for (int j = 0; j < 6; ++j) {
                 const int mode = j < 3 ? bmode : tmode;
                 if (mode != TN_FREESLIP) continue;
@@ -333,8 +506,8 @@ __global__ __launch_bounds__(256) void k_stored_apply(StoredArgs<T, S> a)
                 T Ab[3][3];
                 for (int p = 0; p < 3; ++p)
                     for (int q2 = 0; q2 < 3; ++q2) {
-                        const int r = p * 6 + j, cc = q2 * 6 + j;
-                        Ab[p][q2] = r <= cc ? A(r, cc) : A(cc, r);
+                        const int r = p * 6 + j, cq = q2 * 6 + j;
+                        Ab[p][q2] = r <= cq ? A(r, cq) : A(cq, r);
                     }
                 double n3[3] = {nrm[c][0], nrm[c][1], nrm[c][2]}, t1[3], t2[3];
                 nt_basis_d(n3, t1, t2);
@@ -352,10 +525,8 @@ __global__ __launch_bounds__(256) void k_stored_apply(StoredArgs<T, S> a)
             }
         } else {
             T xm[18];
-#pragma unroll
             for (int i = 0; i < 18; ++i) xm[i] = xs[i];
-            // trial side masks
-#pragma unroll
+            // trial side
             for (int j = 0; j < 6; ++j) {
                 const int mode = j < 3 ? bmode : tmode;
                 if (mode == TN_DIRICHLET) {
@@ -366,19 +537,14 @@ __global__ __launch_bounds__(256) void k_stored_apply(StoredArgs<T, S> a)
                     for (int k = 0; k < 3; ++k) xm[k * 6 + j] -= un * (T)nrm[c][k];
                 }
             }
-#pragma unroll
             for (int i = 0; i < 18; ++i) ys[i] = T(0);
-#pragma unroll
-            for (int i = 0; i < 18; ++i) {
-#pragma unroll
+            for (int i = 0; i < 18; ++i)
                 for (int j = i; j < 18; ++j) {
                     const T aij = A(i, j);
                     ys[i] += aij * xm[j];
                     if (j != i) ys[j] += aij * xm[i];
                 }
-            }
             // test side
-#pragma unroll
             for (int j = 0; j < 6; ++j) {
                 const int mode = j < 3 ? bmode : tmode;
                 if (mode == TN_DIRICHLET) {
@@ -387,12 +553,12 @@ __global__ __launch_bounds__(256) void k_stored_apply(StoredArgs<T, S> a)
                     const int c = WCORN[w][j % 3];
                     T nA[3];  // A_bb n
                     for (int p = 0; p < 3; ++p) {
-                        T s = 0;
+                        T sacc = 0;
                         for (int q2 = 0; q2 < 3; ++q2) {
-                            const int r = p * 6 + j, cc = q2 * 6 + j;
-                            s += (r <= cc ? A(r, cc) : A(cc, r)) * (T)nrm[c][q2];
+                            const int r = p * 6 + j, cq = q2 * 6 + j;
+                            sacc += (r <= cq ? A(r, cq) : A(cq, r)) * (T)nrm[c][q2];
                         }
-                        nA[p] = s;
+                        nA[p] = sacc;
                     }
                     const T ann = (T)nrm[c][0] * nA[0] + (T)nrm[c][1] * nA[1] + (T)nrm[c][2] * nA[2];
                     const T un = (T)nrm[c][0] * xs[j] + (T)nrm[c][1] * xs[6 + j] + (T)nrm[c][2] * xs[12 + j];
@@ -401,36 +567,10 @@ __global__ __launch_bounds__(256) void k_stored_apply(StoredArgs<T, S> a)
                 }
             }
         }
-#pragma unroll
         for (int j = 0; j < 6; ++j) {
             const int c = WCORN[w][j % 3];
-            for (int k = 0; k < 3; ++k) {
-                if (j < 3)
-                    yb[c][k] += ys[k * 6 + j];
-                else
-                    yt[c][k] += ys[k * 6 + j];
-            }
-        }
-    }
-    // radial merge with the previous lane, then scatter
-    const int lane = threadIdx.x & 63;
-    const bool take = active && lane > 0 && rc > 0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const T v = __shfl_up(yt[c][k], 1, 64);
-            if (take) yb[c][k] += v;
-        }
-    if (!active) return;
-    const bool top_flush = lane == 63 || rc == ncr - 1;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int64_t ib = latc[c] * a.nr + rc;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            unsafeAtomicAdd(a.dst + k * N + ib, yb[c][k]);
-            if (top_flush) unsafeAtomicAdd(a.dst + k * N + ib + 1, yt[c][k]);
+            const int64_t ib = latc[c] * a.nr + rc + (j >= 3);
+            for (int k = 0; k < 3; ++k) unsafeAtomicAdd(a.dst + k * N + ib, ys[k * 6 + j]);
         }
     }
 }
@@ -471,13 +611,36 @@ void launch_gca_build(const GcaArgs<T, S, SF>& a, bool fine_geo, hipStream_t s)
 template <typename T, typename S>
 void launch_stored_apply(const StoredArgs<T, S>& a, bool diagonal, hipStream_t s)
 {
-    const int64_t ncell = 10LL * (a.n - 1) * (a.n - 1) * (a.nr - 1);
-    const dim3 gr((unsigned)((ncell + 255) / 256)), bl(256);
+    TN_REQUIRE(a.nr >= 3, "stored apply: at least two radial cell layers");
+    TN_REQUIRE(a.csum && a.ctop, "stored apply: no corner-sum buffers");
+    const int ncr = a.nr - 1, ncl = a.n - 1, nch = (ncr + 63) / 64;
+    const int64_t waves = 10LL * ncl * ncl * nch;
+    const dim3 gr((unsigned)((waves + 3) / 4)), bl(256);
     if (diagonal)
-        hipLaunchKernelGGL((k_stored_apply<T, S, true>), gr, bl, 0, s, a);
+        hipLaunchKernelGGL((k_stored_cells<T, S, true>), gr, bl, 0, s, a);
     else
-        hipLaunchKernelGGL((k_stored_apply<T, S, false>), gr, bl, 0, s, a);
+        hipLaunchKernelGGL((k_stored_cells<T, S, false>), gr, bl, 0, s, a);
     TN_HIP(hipGetLastError());
+    const int64_t N = 10LL * a.n * a.n * a.nr;
+    hipLaunchKernelGGL(k_stored_gather<T>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, (const T*)a.csum,
+                       (const T*)a.ctop, a.dst, a.n, a.nr, a.add);
+    TN_HIP(hipGetLastError());
+    if (a.bc_cmb != TN_NEUMANN || a.bc_surf != TN_NEUMANN) {
+        const int64_t total = 2LL * 10 * (a.n - 1) * (a.n - 1);
+        const dim3 gb((unsigned)((total + 127) / 128)), bb(128);
+        if (diagonal)
+            hipLaunchKernelGGL((k_stored_boundary<T, S, true>), gb, bb, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_stored_boundary<T, S, false>), gb, bb, 0, s, a);
+        TN_HIP(hipGetLastError());
+    }
+}
+
+// elements of the corner-sum buffers: 12 per cell + 12 per (quad, chunk)
+int64_t stored_cellsum_elems(int n, int nr)
+{
+    const int64_t ncol = 10LL * (n - 1) * (n - 1);
+    return 12 * (ncol * (nr - 1) + ncol * ((nr - 1 + 63) / 64));
 }
 
 #define TN_INST_T(T)                                                                                           \

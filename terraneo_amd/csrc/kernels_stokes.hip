@@ -20,6 +20,7 @@
 #include "tn_cell.hpp"
 #include "tn_eps_march.hpp"
 #include "tn_internal.hpp"
+#include "tn_march.hpp"
 #include "tn_stokes.hpp"
 
 namespace tn {
@@ -384,168 +385,93 @@ __global__ __launch_bounds__(256) void k_div(DivArgs<T> a)
     }
 }
 
-// B^T: the atomic-free lateral march of the epsilon kernel over fine cells
-// (a block of W waves owns W-1 node rows; wave w marches fine quad strip
-// x0 - 1 + w in y; lanes are fine radial cells, chunks of 64): each fine cell
-// evaluates its two wedges from the 8 coarse values of its coarse cell and
-// routes the 8 corner sums -- (x, y) complete, (x, y+1) and (x+1, y+1)
-// carried, (x+1, y) through LDS to the next wave, radial levels by a lane
-// shuffle.  Every velocity node copy is written once (store or
-// read-modify-write); Dirichlet shells are left out and free-slip shells
-// projected tangentially on the summed value (the test-side masks and
-// rotations of gradient.hpp:115-325).
-template <typename T, int W>
-__global__ __launch_bounds__(64 * W) void k_grad_march(GradArgs<T> a)
-{
-    constexpr int OWN = W - 1;
-    __shared__ T ebuf[2][W][3][65];
-    __shared__ T lvl[W][MARCH_SEG_MAX][3];
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int n = a.nf, nr = a.nrf, ncr = nr - 1;
-    const int nc = a.nc, nrc = a.nrc;
-    const int nxb = (n + OWN - 1) / OWN, nys = (n + a.seg - 1) / a.seg;
-    int b = blockIdx.x;
-    const int ys = b % nys;
-    b /= nys;
-    const int xb = b % nxb;
-    const int d = b / nxb;
-    const int x = xb * OWN - 1 + w;
-    const bool strip_ok = x >= 0 && x <= n - 2;
-    const bool row_own = w >= 1 && x <= n - 1;
-    const int y0 = ys * a.seg, y1 = min(y0 + a.seg, n);
-    const int64_t N = 10LL * n * n * nr;
-    const int nchunks = (ncr + 63) >> 6;
-    int step = 0;  // LDS double-buffer parity, continuous over chunks
-
-    for (int h = 0; h < nchunks; ++h) {
-        const int c = 64 * h + lane;
-        const bool valid = c < ncr;
-        const int toplane = min(63, ncr - 1 - 64 * h);
-        const bool last = h == nchunks - 1;
-        const int cc = valid ? c : ncr - 1;
-        const RadMoments<T> rm = radial_moments<T>(a.radii[cc], a.radii[cc + 1], cc & 1, valid ? 1.0 : 0.0);
-        const int R = cc >> 1;
-        T c01[3] = {0, 0, 0}, c11[3] = {0, 0, 0}, c01t[3] = {0, 0, 0}, c11t[3] = {0, 0, 0};
-
-        for (int qy = y0 - 1; qy <= y1 - 1; ++qy, ++step) {
-            const bool quad_ok = strip_ok && qy >= 0 && qy <= n - 2;
-            T(*E)[65] = ebuf[step & 1][w];
-            const int64_t lat00 = ((int64_t)d * n + x) * n + qy;
-            T yb[4][3], yt[4][3];
+// B^T: the atomic-free lateral march (tn_march.hpp) over fine cells: each
+// fine cell evaluates its two wedges from the 8 coarse values of its coarse
+// cell.  Every velocity node copy is written once (store or read-modify-
+// write); Dirichlet shells are left out and free-slip shells projected
+// tangentially on the summed value (the test-side masks and rotations of
+// gradient.hpp:115-325).
+template <typename T>
+struct GradCell {
+    const GradArgs<T> a;  // by value: a reference would put the kernel argument in scratch
+    int64_t N;
+    RadMoments<T> rm;
+    int R;
+    __device__ GradCell(const GradArgs<T>& args) : a(args), N(10LL * args.nf * args.nf * args.nrf) {}
+    __device__ void begin_chunk(int, int cc, bool valid)
+    {
+        rm = radial_moments<T>(a.radii[cc], a.radii[cc + 1], cc & 1, valid ? 1.0 : 0.0);
+        R = cc >> 1;
+    }
+    __device__ void cell(int d, int x, int qy, T yb[4][3], T yt[4][3])
+    {
+        const int n = a.nf, nc = a.nc, nrc = a.nrc;
+        const int64_t c00 = ((int64_t)d * nc + (x >> 1)) * nc + (qy >> 1);
+        const int64_t cl[4] = {c00, c00 + nc, c00 + 1, c00 + nc + 1};
+        T pc[4][2];
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 4; ++k) {
+            pc[k][0] = a.p[cl[k] * nrc + R];
+            pc[k][1] = a.p[cl[k] * nrc + R + 1];
+        }
+        const int64_t lat00 = ((int64_t)d * n + x) * n + qy;
+        const int64_t latc[4] = {lat00, lat00 + n, lat00 + 1, lat00 + n + 1};
+        V3<T> P[4];
 #pragma unroll
-                for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
-            if (quad_ok) {
-                const int64_t c00 = ((int64_t)d * nc + (x >> 1)) * nc + (qy >> 1);
-                const int64_t cl[4] = {c00, c00 + nc, c00 + 1, c00 + nc + 1};
-                T pc[4][2];
+        for (int k = 0; k < 4; ++k) P[k] = load_coord<T>(a.coords, latc[k]);
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    pc[k][0] = a.p[cl[k] * nrc + R];
-                    pc[k][1] = a.p[cl[k] * nrc + R + 1];
-                }
-                const int64_t latc[4] = {lat00, lat00 + n, lat00 + 1, lat00 + n + 1};
-                V3<T> P[4];
+        for (int w2 = 0; w2 < 2; ++w2) {
+            const int k0 = WK[w2][0], k1 = WK[w2][1], k2 = WK[w2][2];
+            const LatWedge<T> L = lat_wedge(P[k0], P[k1], P[k2]);
+            const int flw = fine_lateral_wedge_idx(x & 1, qy & 1, w2);
+            T M[3][3];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) P[k] = load_coord<T>(a.coords, latc[k]);
+            for (int il = 0; il < 3; ++il)
 #pragma unroll
-                for (int w2 = 0; w2 < 2; ++w2) {
-                    const int k0 = WK[w2][0], k1 = WK[w2][1], k2 = WK[w2][2];
-                    const LatWedge<T> L = lat_wedge(P[k0], P[k1], P[k2]);
-                    const int flw = fine_lateral_wedge_idx(x & 1, qy & 1, w2);
-                    T M[3][3];
-#pragma unroll
-                    for (int il = 0; il < 3; ++il)
-#pragma unroll
-                        for (int j = 0; j < 3; ++j) M[il][j] = (T)LATM.m[flw][il][j];
-                    const T p[3][2] = {{pc[k0][0], pc[k0][1]}, {pc[k1][0], pc[k1][1]}, {pc[k2][0], pc[k2][1]}};
-                    T o0[2][3] = {{0, 0, 0}, {0, 0, 0}}, o1[2][3] = {{0, 0, 0}, {0, 0, 0}},
-                      o2[2][3] = {{0, 0, 0}, {0, 0, 0}};
-                    grad_wedge(L, M, rm, p, o0, o1, o2);
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        yb[k0][j] += o0[0][j];
-                        yt[k0][j] += o0[1][j];
-                        yb[k1][j] += o1[0][j];
-                        yt[k1][j] += o1[1][j];
-                        yb[k2][j] += o2[0][j];
-                        yt[k2][j] += o2[1][j];
-                    }
-                }
-            }
-            // radial merge (node level c = bottom of cell c + top of cell c-1) and routing
-            T part[3], tp[3] = {0, 0, 0};
+                for (int j = 0; j < 3; ++j) M[il][j] = (T)LATM.m[flw][il][j];
+            const T p[3][2] = {{pc[k0][0], pc[k0][1]}, {pc[k1][0], pc[k1][1]}, {pc[k2][0], pc[k2][1]}};
+            T o0[2][3] = {{0, 0, 0}, {0, 0, 0}}, o1[2][3] = {{0, 0, 0}, {0, 0, 0}}, o2[2][3] = {{0, 0, 0}, {0, 0, 0}};
+            grad_wedge(L, M, rm, p, o0, o1, o2);
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                T Mg[4];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    T up = __shfl_up(yt[k][j], 1, 64);
-                    if (lane == 0) up = T(0);
-                    Mg[k] = yb[k][j] + up;
-                }
-                part[j] = Mg[0] + c01[j];
-                c01[j] = Mg[2];
-                E[j][lane] = Mg[1] + c11[j];
-                c11[j] = Mg[3];
-            }
-            if (lane == toplane) {  // level above this chunk's last cell
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    tp[j] = yt[0][j] + c01t[j];
-                    c01t[j] = yt[2][j];
-                    E[j][64] = yt[1][j] + c11t[j];
-                    c11t[j] = yt[3][j];
-                }
-            }
-            __syncthreads();
-            if (row_own && qy >= y0) {
-                const T(*Ein)[65] = ebuf[step & 1][w - 1];
-                const int64_t lat = lat00;
-                const int slot = qy - y0;
-                auto finish = [&](V3<T> v, int level, int mode) {
-                    const int64_t idx = lat * nr + level;
-                    if (mode == TN_DIRICHLET) {
-                        if (a.add) return;
-                        v = {T(0), T(0), T(0)};
-                    } else if (mode == TN_FREESLIP) {
-                        const V3<T> nn = unit_of(load_coord<T>(a.coords, lat));
-                        const T vn = dot3(nn, v);
-                        v = {v.x - vn * nn.x, v.y - vn * nn.y, v.z - vn * nn.z};
-                    }
-                    if (a.add) {
-                        a.u[idx] += v.x;
-                        a.u[N + idx] += v.y;
-                        a.u[2 * N + idx] += v.z;
-                    } else {
-                        a.u[idx] = v.x;
-                        a.u[N + idx] = v.y;
-                        a.u[2 * N + idx] = v.z;
-                    }
-                };
-                if (valid) {
-                    V3<T> v{part[0] + Ein[0][lane], part[1] + Ein[1][lane], part[2] + Ein[2][lane]};
-                    if (lane == 0 && h > 0) {
-                        v.x += lvl[w][slot][0];
-                        v.y += lvl[w][slot][1];
-                        v.z += lvl[w][slot][2];
-                    }
-                    finish(v, c, c == 0 ? a.bc_cmb : TN_NEUMANN);
-                }
-                if (lane == toplane) {
-                    const V3<T> v{tp[0] + Ein[0][64], tp[1] + Ein[1][64], tp[2] + Ein[2][64]};
-                    if (last) {
-                        finish(v, ncr, a.bc_surf);
-                    } else {
-                        lvl[w][slot][0] = v.x;
-                        lvl[w][slot][1] = v.y;
-                        lvl[w][slot][2] = v.z;
-                    }
-                }
+                yb[k0][j] += o0[0][j];
+                yt[k0][j] += o0[1][j];
+                yb[k1][j] += o1[0][j];
+                yt[k1][j] += o1[1][j];
+                yb[k2][j] += o2[0][j];
+                yt[k2][j] += o2[1][j];
             }
         }
     }
+    __device__ void finish(int64_t lat, int level, V3<T> v)
+    {
+        const int mode = level == 0 ? a.bc_cmb : (level == a.nrf - 1 ? a.bc_surf : TN_NEUMANN);
+        const int64_t idx = lat * a.nrf + level;
+        if (mode == TN_DIRICHLET) {
+            if (a.add) return;
+            v = {T(0), T(0), T(0)};
+        } else if (mode == TN_FREESLIP) {
+            const V3<T> nn = unit_of(load_coord<T>(a.coords, lat));
+            const T vn = dot3(nn, v);
+            v = {v.x - vn * nn.x, v.y - vn * nn.y, v.z - vn * nn.z};
+        }
+        if (a.add) {
+            a.u[idx] += v.x;
+            a.u[N + idx] += v.y;
+            a.u[2 * N + idx] += v.z;
+        } else {
+            a.u[idx] = v.x;
+            a.u[N + idx] = v.y;
+            a.u[2 * N + idx] = v.z;
+        }
+    }
+};
+
+template <typename T, int W>
+__global__ __launch_bounds__(64 * W) void k_grad_march(GradArgs<T> a)
+{
+    GradCell<T> op(a);
+    march::run<T, W>(op, a.nf, a.nrf, a.seg);
 }
 
 // VectorMass (ncomp components) / lumped k-weighted mass (LUMPED_K).
@@ -630,8 +556,8 @@ void launch_grad(const GradArgs<T>& a, hipStream_t s)
                "B^T: velocity mesh must be the pressure mesh refined once");
     TN_REQUIRE(a.seg >= 1 && a.seg <= MARCH_SEG_MAX, "B^T: segment length out of range");
     constexpr int W = GRAD_MARCH_WAVES;
-    const int nxb = (a.nf + W - 2) / (W - 1), nys = (a.nf + a.seg - 1) / a.seg;
-    hipLaunchKernelGGL((k_grad_march<T, W>), dim3((unsigned)(10 * nxb * nys)), dim3(64 * W), 0, s, a);
+    hipLaunchKernelGGL((k_grad_march<T, W>), dim3((unsigned)march::blocks_for(a.nf, a.seg, W)), dim3(64 * W), 0, s,
+                       a);
     TN_HIP(hipGetLastError());
 }
 

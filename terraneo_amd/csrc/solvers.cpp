@@ -516,6 +516,12 @@ void do_prolongate(const Transfer& t, const BVec& xc, const BVec& xf, hipStream_
 {
     launch_prolongate_add<T>(*t.coarse, *t.fine, xc.b[0].ncomp, (const T*)xc.b[0].p, (T*)xf.b[0].p, s);
 }
+template <typename T>
+void fine_project(const Transfer& t, const BVec& v, hipStream_t s)
+{
+    if (t.project_cmb) launch_freeslip_project<T>(*t.fine, (T*)v.b[0].p, 1, s);
+    if (t.project_surf) launch_freeslip_project<T>(*t.fine, (T*)v.b[0].p, 2, s);
+}
 }  // namespace
 
 void Multigrid::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
@@ -534,16 +540,23 @@ void Multigrid::recurse(Ctx& c, LinOp& A, const BVec& x, const BVec& b, int leve
     A.apply(c, x, tmp_[level]);
     v_lincomb(c, tmp_[level], {1.0, -1.0}, {&b, &tmp_[level]});
     const Transfer& t = tr_[level - 1];
-    if (x.b[0].dtype == TN_F64)
+    const bool f64 = x.b[0].dtype == TN_F64;
+    if (f64) {
+        fine_project<double>(t, tmp_[level], c.s);
         do_restrict<double>(t, tmp_[level], r_[level - 1], c.s);
-    else
+    } else {
+        fine_project<float>(t, tmp_[level], c.s);
         do_restrict<float>(t, tmp_[level], r_[level - 1], c.s);
+    }
     v_fill(c, e_[level - 1], 0.0);
     recurse(c, *ops_[level - 1], e_[level - 1], r_[level - 1], level - 1);
-    if (x.b[0].dtype == TN_F64)
+    if (f64) {
         do_prolongate<double>(t, e_[level - 1], x, c.s);
-    else
+        fine_project<double>(t, x, c.s);
+    } else {
         do_prolongate<float>(t, e_[level - 1], x, c.s);
+        fine_project<float>(t, x, c.s);
+    }
     sm_[level]->solve(c, A, x, b);
 }
 

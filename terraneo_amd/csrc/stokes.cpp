@@ -117,6 +117,18 @@ __global__ void k_freeslip_project(T* v, const double* coords, int64_t N, int nr
 
 }  // namespace
 
+namespace tn {
+template <typename T>
+void launch_freeslip_project(const MeshLevel& m, T* v, int which, hipStream_t s)
+{
+    const unsigned g = (unsigned)((m.N + 255) / 256);
+    hipLaunchKernelGGL(k_freeslip_project<T>, dim3(g), dim3(256), 0, s, v, m.d_coords.p, m.N, m.nr, which);
+    TN_HIP(hipGetLastError());
+}
+template void launch_freeslip_project<double>(const MeshLevel&, double*, int, hipStream_t);
+template void launch_freeslip_project<float>(const MeshLevel&, float*, int, hipStream_t);
+}  // namespace tn
+
 extern "C" {
 
 int tn_stokes_create(const tn_mesh* vel, const tn_mesh* pre, tn_eps* A, tn_stokes** out)
@@ -261,14 +273,10 @@ int tn_freeslip_project(const tn_mesh* m, int dtype, void* v, int which, tn_stre
     TN_API_BEGIN
     TN_REQUIRE(m && v, "null pointer");
     auto& L = const_cast<tn_mesh*>(m)->lvl.dev();
-    const unsigned g = (unsigned)((L.N + 255) / 256);
     if (dtype == TN_F64)
-        hipLaunchKernelGGL(k_freeslip_project<double>, dim3(g), dim3(256), 0, (hipStream_t)s, (double*)v,
-                           L.d_coords.p, L.N, L.nr, which);
+        tn::launch_freeslip_project<double>(L, (double*)v, which, (hipStream_t)s);
     else
-        hipLaunchKernelGGL(k_freeslip_project<float>, dim3(g), dim3(256), 0, (hipStream_t)s, (float*)v,
-                           L.d_coords.p, L.N, L.nr, which);
-    TN_HIP(hipGetLastError());
+        tn::launch_freeslip_project<float>(L, (float*)v, which, (hipStream_t)s);
     TN_API_END
 }
 

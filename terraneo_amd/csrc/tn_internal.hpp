@@ -130,6 +130,10 @@ void launch_lincomb(int64_t count, T* y, int nx, const double* c, const T* const
 // out_dev[0..nd) += sum over owned nodes, accumulate in double; ncomp fields of N each
 template <typename T>
 void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, double* out_dev, hipStream_t s);
+// tangential projection of a vector field's nodes on shell `which` (1 CMB, 2 surface)
+template <typename T>
+void launch_freeslip_project(const MeshLevel& m, T* v, int which, hipStream_t s);
+
 // y[i] -= scale * sum_dev[0], i < count
 template <typename T>
 void launch_sub_scaled(T* y, int64_t count, const double* sum_dev, double scale, hipStream_t s);

@@ -38,12 +38,15 @@ __device__ inline void nt_basis_d(const double n[3], double t1[3], double t2[3])
 
 template <typename T, typename S>
 struct StoredArgs {
-    const S* stored;       // [2*171][cells]
+    const S* stored;       // stored_at() layout
     const T* src;
     T* dst;
     const double* coords;  // [10][n][n][3]
     int n, nr;
     int bc_cmb, bc_surf;
+    int add;     // 1: dst += A x, 0: dst = A x
+    T* csum;     // [12][cells] merged corner sums (pass 1 -> pass 2)
+    T* ctop;     // [12][10 (n-1)^2 chunks] chunk-top level sums
 };
 
 template <typename T, typename S, typename SF>
@@ -60,6 +63,30 @@ struct GcaArgs {
 
 constexpr int PACKED = 171;
 
+// Stored-matrix layout.  Each wedge's 171 packed entries are padded to
+// WEDGE_SLOTS = 172 slots (slot w*172 + k), cells (diamond, x, y, r order, r
+// fastest) are grouped in tiles of 64, and inside a tile the slots are
+// interleaved in 16-byte vectors of V = 16 / sizeof(S) slots per lane:
+//   element (cell c, slot e) at ((c/64 * G + e/V) * 64 + c%64) * V + e%V,
+//   G = 344 / V vectors per cell.
+// A wave whose 64 lanes are 64 radially consecutive cells reads its quad's
+// matrices as G coalesced 1 KB vector loads (one per lane per instruction).
+constexpr int WEDGE_SLOTS = 172;
+template <typename S>
+__host__ __device__ constexpr int stored_vec()
+{
+    return 16 / (int)sizeof(S);
+}
+template <typename S>
+__host__ __device__ inline int64_t stored_at(int64_t cell, int w, int k)
+{
+    constexpr int V = stored_vec<S>(), G = 2 * WEDGE_SLOTS / V;
+    const int e = w * WEDGE_SLOTS + k;
+    return (((cell >> 6) * G + e / V) * 64 + (cell & 63)) * V + e % V;
+}
+// elements to allocate for `cells` cells
+inline int64_t stored_elems(int64_t cells) { return ((cells + 63) >> 6) * 64 * 2 * WEDGE_SLOTS; }
+
 template <typename T>
 void launch_prolongate_add(const MeshLevel& mc, const MeshLevel& mf, int ncomp, const T* xc, T* xf, hipStream_t s);
 template <typename T>
@@ -68,5 +95,7 @@ template <typename T, typename S, typename SF>
 void launch_gca_build(const GcaArgs<T, S, SF>& a, bool fine_geo, hipStream_t s);
 template <typename T, typename S>
 void launch_stored_apply(const StoredArgs<T, S>& a, bool diagonal, hipStream_t s);
+// elements of StoredArgs::csum + ctop for an n x n x nr mesh
+int64_t stored_cellsum_elems(int n, int nr);
 
 }  // namespace tn

@@ -21,7 +21,10 @@ struct tn_eps {
     bool blk_ready = false;
     // stored local matrices (Galerkin coarse operators), packed symmetric
     int stored_dtype = -1;          // -1: matrix-free
-    tn::DevBuf<unsigned char> stored;  // [2*171][cells] of stored_dtype
+    tn::DevBuf<unsigned char> stored;  // tn::stored_at layout, stored_dtype elements
+    // stored apply: per-cell corner sums between its two passes (12 per cell
+    // plus 12 per radial chunk top), allocated on first use
+    tn::DevBuf<unsigned char> cellsum;
     int64_t cells() const
     {
         const auto& m = mesh->lvl;

@@ -172,6 +172,8 @@ class FGMRES : public Solver {
 struct Transfer {
     MeshLevel* coarse;
     MeshLevel* fine;
+    // free-slip projected transfer on the fine level's shells (1 CMB, 2 surface)
+    bool project_cmb = false, project_surf = false;
 };
 
 class Multigrid : public Solver {

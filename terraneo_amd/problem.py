@@ -31,7 +31,8 @@ class StokesProblem:
     levels (the coarsest has lateral level lat-levels+1); bc: (CMB, surface)."""
 
     def __init__(self, lat, rad, levels, r_min=0.55, r_max=1.0, bc=(TN_FREESLIP, TN_FREESLIP), device="cuda:0",
-                 prec_dtype="f64", coarsest_f64=True, gmg=None, gca_bracket="reference", coarse_ops="gca"):
+                 prec_dtype="f64", coarsest_f64=True, gmg=None, gca_bracket="reference", coarse_ops="gca",
+                 mg_penalty=None):
         torch = _torch()
         assert levels >= 2 and lat - levels + 1 >= 0 and rad - levels + 1 >= 1
         self.device = torch.device(device)
@@ -47,6 +48,13 @@ class StokesProblem:
         # "gca": Galerkin matrices (gca.hpp); "rediscretize": matrix-free coarse
         # operators with the nodal viscosity injected (the assess tests' setup)
         self.coarse_ops = coarse_ops
+        # fs/fs rotation penalty of the multigrid level operators: None keeps the
+        # operator's epsilon (1, the reference's); a number s sets epsilon to s
+        # times the level's mean diagonal entry, so that the penalised rotation
+        # modes sit inside the spectrum the Chebyshev smoother targets (with
+        # epsilon = 1 they leave it as the mesh is refined).  The outer Stokes
+        # operator always keeps epsilon = 1.
+        self.mg_penalty = mg_penalty
         self.gmg_opts = gmg or api.gmg_options(cheby_order=3, cheby_iterations=1, power_iterations=10, num_cycles=1)
         self.f64 = dict(dtype=torch.float64, device=self.device)
         self.A = None
@@ -74,7 +82,7 @@ class StokesProblem:
         if pd == torch.float64:
             self._eta_p = self.eta
             self._eta_cell_p = self.eta_cell
-            fine = self.A
+            fine = self.A if self.mg_penalty is None else api.Eps(self.vel, self.eta, *bc, eta_cell=self.eta_cell)
         else:
             self._eta_p = self.eta.to(pd) if self.eta is not None else None
             self._eta_cell_p = self.eta_cell.to(pd) if self.eta_cell is not None else None
@@ -100,10 +108,24 @@ class StokesProblem:
                 op.gca_from(ops[-1], sdt, bracket=self.gca_bracket)
             ops.append(op)
         self.ops = list(reversed(ops))
+        if self.mg_penalty is not None and bc == (TN_FREESLIP, TN_FREESLIP):
+            for op, m in zip(self.ops, self.meshes):
+                op.set_penalty_epsilon(float(self.mg_penalty) * self._mean_diagonal(op, m))
         self.gmg = api.Gmg(self.ops, self.gmg_opts)
         # Schur preconditioner: lumped M_p(1/eta), eta sampled at the pressure nodes
         self.schur = torch.empty(self.pre.N, **self.f64)
         api.kmass_lumped(self.pre, (1.0 / self.pressure_viscosity()).contiguous(), self.schur)
+
+    def _mean_diagonal(self, op, mesh):
+        """Mean over owned velocity dofs of the operator's diagonal."""
+        torch = _torch()
+        dt = torch.float64 if op.dtype == TN_F64 else torch.float32
+        one = torch.ones(3 * mesh.N, dtype=dt, device=self.device)
+        d = torch.empty_like(one)
+        op.set_diagonal(True)
+        op.apply(one, d)
+        op.set_diagonal(False)
+        return float(mesh.dot(d, one, 3)) / (3 * mesh.owned_nodes)
 
     def pressure_viscosity(self):
         """Viscosity at the pressure nodes (the velocity nodes of even index)."""
@@ -154,12 +176,17 @@ class StokesProblem:
                                 rtol=rtol, atol=1e-30, restart=restart, pressure_mean_free=pressure_mean_free)
         return u, p, info
 
-    def residual(self, u, p, bu, bp):
-        """Relative residual ||b - K x|| / ||b|| in the owned-node norm."""
+    def residual(self, u, p, bu, bp, pressure_mean_free=False):
+        """Relative residual ||b - K x|| / ||b|| in the owned-node norm
+        (pressure_mean_free: of the mean-projected system, i.e. with the
+        pressure residual's owned mean removed)."""
         torch = _torch()
         yu, yp = torch.empty_like(u), torch.empty_like(p)
         self.K.apply(u, p, yu, yp)
         ru, rp = bu - yu, bp - yp
+        if pressure_mean_free:
+            one = torch.ones_like(rp)
+            rp = rp - self.pre.dot(rp, one, 1) / self.pre.dot(one, one, 1)
         num = self.vel.dot(ru, ru, 3) + self.pre.dot(rp, rp, 1)
         den = self.vel.dot(bu, bu, 3) + self.pre.dot(bp, bp, 1)
         return float(np.sqrt(num / den))
