@@ -65,6 +65,11 @@ typedef struct tn_gmg_options {
      * Pi the free-slip projection): the constant prolongation of a coarse
      * tangential field is not tangential at the fine boundary nodes. */
     int freeslip_projected_transfer;
+    /* 0: the reference's Jacobi scaling, the entrywise inverse of the
+     * operator's diagonal.  1: at free-slip node copies, the inverse of the
+     * operator's 3x3 free-slip diagonal R^T diag(R B R^T) R instead (its
+     * entrywise inversion is not that inverse). */
+    int freeslip_block_jacobi;
 } tn_gmg_options;
 
 /* coarsest-level solver of the multigrid: unpreconditioned PCG as the
@@ -145,6 +150,18 @@ int tn_lincomb(const tn_mesh* m, int dtype, int ncomp, void* y, int nx, const do
                const void* const* x, double c0, tn_stream_t s);
 /* result written to host *out (synchronises the stream) */
 int tn_dot(const tn_mesh* m, int dtype, int ncomp, const void* a, const void* b, double* out, tn_stream_t s);
+/* v = 1 / v entrywise (invert_entries, grid_operations.hpp:252-270) */
+int tn_invert_entries(const tn_mesh* m, int dtype, int ncomp, void* v, tn_stream_t s);
+/* y = y * d entrywise (scale_with_vector / mult, grid_operations.hpp:272-294) */
+int tn_scale_with_vector(const tn_mesh* m, int dtype, int ncomp, void* y, const void* d, tn_stream_t s);
+/* uniform [0, 1) entries, each stored node copy drawn independently like the
+ * reference's rand (grid_operations.hpp:758-783); counter-based (SplitMix64 of
+ * seed and index), so the values differ from Kokkos' random pool */
+int tn_randomize(const tn_mesh* m, int dtype, int ncomp, void* y, uint64_t seed, tn_stream_t s);
+/* max |v| over all stored entries (max_abs_entry, grid_operations.hpp:336-368); host *out */
+int tn_max_abs_entry(const tn_mesh* m, int dtype, int ncomp, const void* v, double* out, tn_stream_t s);
+/* *out = 1 if any entry is NaN or infinite (has_nan_or_inf, grid_operations.hpp:713-743) */
+int tn_has_nan_or_inf(const tn_mesh* m, int dtype, int ncomp, const void* v, int* out, tn_stream_t s);
 
 /* ------------------------------------------------------------------ epsilon-div-div operator
  * EpsilonDivDivKerngen<T> (src/terra/fe/wedge/operators/shell/epsilon_divdiv_kerngen.hpp:61-2112):

@@ -20,7 +20,7 @@ TN_GCA_REFERENCE, TN_GCA_GEOMETRIC = 0, 1
 class GmgOptions(C.Structure):
     _fields_ = [("cheby_order", C.c_int), ("cheby_iterations", C.c_int), ("power_iterations", C.c_int),
                 ("num_cycles", C.c_int), ("coarse_max_iterations", C.c_int), ("coarse_rtol", C.c_double),
-                ("coarse_atol", C.c_double), ("seed", C.c_uint64), ("coarse_solver", C.c_int), ("freeslip_projected_transfer", C.c_int)]
+                ("coarse_atol", C.c_double), ("seed", C.c_uint64), ("coarse_solver", C.c_int), ("freeslip_projected_transfer", C.c_int), ("freeslip_block_jacobi", C.c_int)]
 
 
 class KrylovOptions(C.Structure):
@@ -65,6 +65,11 @@ SIGNATURES = {
     "tn_interface_sum": (i32, [vp, i32, i32, vp, vp]),
     "tn_lincomb": (i32, [vp, i32, i32, vp, i32, C.POINTER(f64), pvp, f64, vp]),
     "tn_dot": (i32, [vp, i32, i32, vp, vp, C.POINTER(f64), vp]),
+    "tn_invert_entries": (i32, [vp, i32, i32, vp, vp]),
+    "tn_scale_with_vector": (i32, [vp, i32, i32, vp, vp, vp]),
+    "tn_randomize": (i32, [vp, i32, i32, vp, C.c_uint64, vp]),
+    "tn_max_abs_entry": (i32, [vp, i32, i32, vp, C.POINTER(f64), vp]),
+    "tn_has_nan_or_inf": (i32, [vp, i32, i32, vp, C.POINTER(C.c_int), vp]),
     "tn_eps_create": (i32, [vp, i32, vp, i32, i32, pvp]),
     "tn_eps_destroy": (i32, [vp]),
     "tn_eps_set_modes": (i32, [vp, i32, i32]),

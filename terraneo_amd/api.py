@@ -94,6 +94,25 @@ class Mesh:
         x = (C.c_void_p * max(nx, 1))(*[t.data_ptr() for t in xs])
         check(lib().tn_lincomb(self.h, dtype_code(y.dtype), ncomp, _ptr(y), nx, c, x, c0, _stream_ptr(stream)))
 
+    def invert_entries(self, v, ncomp=3, stream=None):
+        check(lib().tn_invert_entries(self.h, dtype_code(v.dtype), ncomp, _ptr(v), _stream_ptr(stream)))
+
+    def scale_with_vector(self, y, d, ncomp=3, stream=None):
+        check(lib().tn_scale_with_vector(self.h, dtype_code(y.dtype), ncomp, _ptr(y), _ptr(d), _stream_ptr(stream)))
+
+    def randomize(self, y, seed, ncomp=3, stream=None):
+        check(lib().tn_randomize(self.h, dtype_code(y.dtype), ncomp, _ptr(y), seed, _stream_ptr(stream)))
+
+    def max_abs_entry(self, v, ncomp=3, stream=None):
+        out = C.c_double()
+        check(lib().tn_max_abs_entry(self.h, dtype_code(v.dtype), ncomp, _ptr(v), C.byref(out), _stream_ptr(stream)))
+        return out.value
+
+    def has_nan_or_inf(self, v, ncomp=3, stream=None):
+        out = C.c_int()
+        check(lib().tn_has_nan_or_inf(self.h, dtype_code(v.dtype), ncomp, _ptr(v), C.byref(out), _stream_ptr(stream)))
+        return bool(out.value)
+
     def __del__(self):
         try:
             if self.h:
@@ -235,10 +254,10 @@ class Stokes:
 
 def gmg_options(cheby_order=2, cheby_iterations=3, power_iterations=10, num_cycles=1, coarse_max_iterations=1000,
                 coarse_rtol=1e-6, coarse_atol=1e-16, seed=12345, coarse_solver="pcg",
-                freeslip_projected_transfer=False):
+                freeslip_projected_transfer=False, freeslip_block_jacobi=False):
     cs = {"pcg": 0, "dense": 1}[coarse_solver]
     return GmgOptions(cheby_order, cheby_iterations, power_iterations, num_cycles, coarse_max_iterations, coarse_rtol,
-                      coarse_atol, seed, cs, int(bool(freeslip_projected_transfer)))
+                      coarse_atol, seed, cs, int(bool(freeslip_projected_transfer)), int(bool(freeslip_block_jacobi)))
 
 
 class Gmg:

@@ -163,22 +163,32 @@ void eps_apply_t(tn_eps* op, const void* src_v, void* dst_v, hipStream_t s)
                            m.n, m.nr, march_seg(m.n, (int)sizeof(T)), op->bc_cmb, op->bc_surf, add ? 1 : 0};
         tn::launch_eps_march<T>(a, op->diagonal, s);
     } else {
-        // the gather writes every node copy once: the penalty term first, then accumulate
-        if (penalty) tn::launch_penalty_init<T>(m, op->pen.p, op->pen.p + 4, dst, add, s);
-        const int acc = (add || penalty) ? 1 : 0;
+        // the gather writes every node copy once (boundary test side and penalty fused)
         const int64_t ncs = tn::stored_cellsum_elems(m.n, m.nr);
         if (op->cellsum.n != (size_t)ncs * sizeof(T)) op->cellsum.alloc((size_t)ncs * sizeof(T));
         T* csum = (T*)op->cellsum.p;
         T* ctop = csum + 12 * (10LL * (m.n - 1) * (m.n - 1) * (m.nr - 1));
-        if (op->stored_dtype == TN_F64) {
-            tn::StoredArgs<T, double> a{(const double*)op->stored.p, src, dst, m.d_coords.p, m.n, m.nr, op->bc_cmb,
-                                        op->bc_surf, acc, csum, ctop};
-            tn::launch_stored_apply<T, double>(a, op->diagonal, s);
-        } else {
-            tn::StoredArgs<T, float> a{(const float*)op->stored.p, src, dst, m.d_coords.p, m.n, m.nr, op->bc_cmb,
-                                       op->bc_surf, acc, csum, ctop};
-            tn::launch_stored_apply<T, float>(a, op->diagonal, s);
+        const bool need_blk = op->bc_cmb != TN_NEUMANN || op->bc_surf != TN_NEUMANN;
+        const size_t blk_bytes = sizeof(T) * 2 * 10 * (size_t)m.n * m.n * 3;
+        if (need_blk && op->blk.n != blk_bytes) {
+            op->blk.alloc(blk_bytes);
+            op->blk_ready = false;
         }
+        auto run = [&](auto* stored) {
+            using S = std::remove_const_t<std::remove_pointer_t<decltype(stored)>>;
+            tn::StoredArgs<T, S> a{stored, src, dst, m.d_coords.p, m.n, m.nr, op->bc_cmb, op->bc_surf, add ? 1 : 0,
+                                   csum, ctop, need_blk ? (const T*)op->blk.p : nullptr,
+                                   penalty ? op->pen.p : nullptr, m.d_radii.p};
+            if (need_blk && !op->blk_ready) {
+                tn::launch_stored_blk<T, S>(a, (T*)op->blk.p, s);
+                op->blk_ready = true;
+            }
+            tn::launch_stored_apply<T, S>(a, op->diagonal, s);
+        };
+        if (op->stored_dtype == TN_F64)
+            run((const double*)op->stored.p);
+        else
+            run((const float*)op->stored.p);
     }
     if (op->comm_mode == TN_COMM_ADDITIVE) tn::launch_interface_sum<T>(m, 3, dst, s);
 }
@@ -361,6 +371,75 @@ int tn_dot(const tn_mesh* m, int dtype, int ncomp, const void* a, const void* b,
     TN_API_END
 }
 
+int tn_invert_entries(const tn_mesh* m, int dtype, int ncomp, void* v, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && v, "null pointer");
+    const int64_t cnt = m->lvl.N * ncomp;
+    if (dtype == TN_F64)
+        tn::launch_invert<double>((double*)v, cnt, (hipStream_t)s);
+    else
+        tn::launch_invert<float>((float*)v, cnt, (hipStream_t)s);
+    TN_API_END
+}
+
+int tn_scale_with_vector(const tn_mesh* m, int dtype, int ncomp, void* y, const void* d, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && y && d, "null pointer");
+    const int64_t cnt = m->lvl.N * ncomp;
+    if (dtype == TN_F64)
+        tn::launch_mul<double>((double*)y, (const double*)d, cnt, (hipStream_t)s);
+    else
+        tn::launch_mul<float>((float*)y, (const float*)d, cnt, (hipStream_t)s);
+    TN_API_END
+}
+
+int tn_randomize(const tn_mesh* m, int dtype, int ncomp, void* y, uint64_t seed, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && y, "null pointer");
+    const int64_t cnt = m->lvl.N * ncomp;
+    if (dtype == TN_F64)
+        tn::launch_randomize<double>((double*)y, cnt, seed, (hipStream_t)s);
+    else
+        tn::launch_randomize<float>((float*)y, cnt, seed, (hipStream_t)s);
+    TN_API_END
+}
+
+int tn_max_abs_entry(const tn_mesh* m, int dtype, int ncomp, const void* v, double* out, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && v && out, "null pointer");
+    auto& L = const_cast<tn_mesh*>(m)->lvl.dev();
+    const int64_t cnt = L.N * ncomp;
+    tn::launch_fill<double>(L.d_scratch.p, 1, 0.0, (hipStream_t)s);
+    if (dtype == TN_F64)
+        tn::launch_max_abs<double>((const double*)v, cnt, L.d_scratch.p, (hipStream_t)s);
+    else
+        tn::launch_max_abs<float>((const float*)v, cnt, L.d_scratch.p, (hipStream_t)s);
+    TN_HIP(hipMemcpyAsync(out, L.d_scratch.p, sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)s));
+    TN_HIP(hipStreamSynchronize((hipStream_t)s));
+    TN_API_END
+}
+
+int tn_has_nan_or_inf(const tn_mesh* m, int dtype, int ncomp, const void* v, int* out, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && v && out, "null pointer");
+    auto& L = const_cast<tn_mesh*>(m)->lvl.dev();
+    const int64_t cnt = L.N * ncomp;
+    int* flag = reinterpret_cast<int*>(L.d_scratch.p);
+    tn::launch_fill<int>(flag, 1, 0, (hipStream_t)s);
+    if (dtype == TN_F64)
+        tn::launch_nonfinite<double>((const double*)v, cnt, flag, (hipStream_t)s);
+    else
+        tn::launch_nonfinite<float>((const float*)v, cnt, flag, (hipStream_t)s);
+    TN_HIP(hipMemcpyAsync(out, flag, sizeof(int), hipMemcpyDeviceToHost, (hipStream_t)s));
+    TN_HIP(hipStreamSynchronize((hipStream_t)s));
+    TN_API_END
+}
+
 // ------------------------------------------------------------------ epsilon operator
 int tn_eps_create(const tn_mesh* m, int dtype, const void* eta, int bc_cmb, int bc_surf, tn_eps** out)
 {
@@ -486,6 +565,7 @@ int tn_eps_gca_from_bracket(tn_eps* coarse, const tn_eps* fine, int store_dtype,
     const int64_t bytes = tn::stored_elems(coarse->cells()) * (store_dtype == TN_F64 ? 8 : 4);
     coarse->stored.alloc((size_t)bytes);
     coarse->stored_dtype = store_dtype;
+    coarse->blk_ready = false;  // boundary forms follow the new matrices
     const bool fine_geo = fine->stored_dtype < 0;
     TN_REQUIRE(!fine_geo || fine->eta || fine->eta_cell, "GCA: fine operator has no viscosity");
     auto go = [&](auto* storec, auto* eta_f, auto* stored_f) {

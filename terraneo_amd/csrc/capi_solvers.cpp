@@ -4,6 +4,7 @@
 #include <memory>
 
 #include "tn_coarse.hpp"
+#include "tn_mg.hpp"
 #include "tn_objects.hpp"
 #include "tn_solvers.hpp"
 
@@ -172,6 +173,20 @@ void gmg_setup(tn_gmg* g, const tn_gmg_options& o, hipStream_t s)
         }
         g->sm.emplace_back(new Chebyshev(o.cheby_order, o.cheby_iterations, g->invd.back(), g->ta.back(), g->tb.back(),
                                          o.power_iterations, o.seed + 1000003ull * l));
+        if (o.freeslip_block_jacobi && (op->bc_cmb == TN_FREESLIP || op->bc_surf == TN_FREESLIP)) {
+            // the diagonal apply above computed the operator's boundary forms (op->blk)
+            TN_REQUIRE(op->blk_ready, "free-slip block Jacobi: boundary forms missing");
+            MeshLevel* ml = &const_cast<tn_mesh*>(op->mesh)->lvl.dev();
+            const BVec dinv = g->invd.back();
+            g->sm.back()->scale = [ml, op, dinv](Ctx& cc, const BVec& z) {
+                if (z.b[0].dtype == TN_F64)
+                    launch_fs_block_scale<double>(*ml, (double*)z.b[0].p, (const double*)dinv.b[0].p,
+                                                  (const double*)op->blk.p, op->bc_cmb, op->bc_surf, cc.s);
+                else
+                    launch_fs_block_scale<float>(*ml, (float*)z.b[0].p, (const float*)dinv.b[0].p,
+                                                 (const float*)op->blk.p, op->bc_cmb, op->bc_surf, cc.s);
+            };
+        }
     }
     const BVec like0 = vec1(field_of(g->ops[0]->mesh, g->ops[0]->dtype, 3, nullptr));
     for (int i = 0; i < 4; ++i) g->ctmp.push_back(g->store.make(like0));

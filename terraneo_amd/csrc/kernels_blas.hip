@@ -204,6 +204,18 @@ __global__ void k_nonfinite(const T* __restrict__ y, int64_t count, int* flag)
     if (bad) atomicOr(flag, 1);
 }
 
+// out = max |y[i]| (non-negative doubles order like their bit patterns)
+template <typename T>
+__global__ __launch_bounds__(256) void k_max_abs(const T* __restrict__ y, int64_t count, unsigned long long* out)
+{
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs((double)y[i]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
 // out[k] += sum_owned a . b_k for k < nb (multi-dot, one pass over a)
 template <typename T, int NB>
 __global__ __launch_bounds__(256) void k_multidot(const T* __restrict__ a, MultiDotArgs<T> bs,
@@ -348,6 +360,14 @@ void launch_nonfinite(const T* y, int64_t count, int* flag_dev, hipStream_t s)
 }
 
 template <typename T>
+void launch_max_abs(const T* y, int64_t count, double* out_dev, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_max_abs<T>, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, y, count,
+                       reinterpret_cast<unsigned long long*>(out_dev));
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
 void launch_multidot(const MeshLevel& m, int ncomp, const T* a, const MultiDotArgs<T>& bs, double* out_dev,
                      hipStream_t s)
 {
@@ -375,6 +395,7 @@ template void launch_cast<float, double>(float*, const double*, int64_t, hipStre
     template void launch_penalty_moment<T>(const MeshLevel&, const T*, double*, hipStream_t);                    \
     template void launch_penalty_init<T>(const MeshLevel&, const double*, const double*, T*, bool, hipStream_t); \
     template void launch_mul<T>(T*, const T*, int64_t, hipStream_t);                                             \
+    template void launch_max_abs<T>(const T*, int64_t, double*, hipStream_t);                                    \
     template void launch_invert<T>(T*, int64_t, hipStream_t);                                                    \
     template void launch_randomize<T>(T*, int64_t, uint64_t, hipStream_t);                                       \
     template void launch_nonfinite<T>(const T*, int64_t, int*, hipStream_t);                                     \

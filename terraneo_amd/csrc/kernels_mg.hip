@@ -304,29 +304,31 @@ __global__ void k_gca_build(GcaArgs<T, S, SF> a)
 
 // ---------------------------------------------------------------- stored-matrix apply
 //
-// Three launches, every node copy finished once, deterministic:
+// Two launches, every node copy finished once, no atomics, deterministic:
 //  * k_stored_cells: a wave per lateral quad and chunk of 64 radial cells
-//    (one per lane).  Each cell not in a Dirichlet / free-slip boundary layer
-//    applies its two wedge matrices (plain product, or their diagonal) read
-//    as coalesced 16-byte vectors of the tile layout (stored_at); the cell's
-//    8 corner sums are merged radially with a lane shuffle (node level c =
-//    bottom of cell c + top of cell c-1) and the 4 lateral corners x 3
-//    components at level c are written per cell, the chunk's top level once
-//    per chunk.  No barriers, no atomics: the matrix stream runs at the
-//    occupancy the registers allow.
+//    (one per lane).  Each cell applies its two wedge matrices (plain
+//    product, or their diagonal) read as coalesced 16-byte vectors of the tile
+//    layout (stored_at); the cell's 8 corner sums are merged radially with a
+//    lane shuffle (node level c = bottom of cell c + top of cell c-1) and the
+//    4 lateral corners x 3 components at level c are written per cell, the
+//    chunk's top level once per chunk.  No barriers: the matrix stream runs at
+//    the occupancy the registers allow.
 //  * k_stored_gather: a thread per node copy sums its (at most 4) lateral
 //    quads' corner values and stores (Replace) or adds (Add) the result.
-//  * k_stored_boundary: the (at most two) boundary cell layers with the
-//    boundary treatment the reference applies to its stored matrices
-//    (epsilon_divdiv_kerngen.hpp:823-1051) -- trial side masked (Dirichlet)
-//    or projected (free-slip), test side replaced by the Dirichlet diagonal or
-//    completed by (n^T A_bb n)(n . x) n; diagonal mode keeps diag(A) and, at
-//    free-slip nodes, R^T diag(R A_bb R^T) R -- added with atomics after the
-//    gather (2 of the nr-1 layers; that code stays out of the streaming
-//    kernel's register budget).
+// Boundary conditions are algebraic, as in the matrix-free apply: on a
+// Dirichlet / free-slip shell the cells mask / project the trial side, the
+// gather completes the test side from the node's boundary forms (k_stored_blk,
+// cached per operator) and adds the fs/fs penalty.  This is the matrix the
+// reference applies to its stored matrices through masks and rotations
+// (epsilon_divdiv_kerngen.hpp:823-1051).
+// Trial side on the boundary level of this lane's cell, branch-free (a
+// divergent branch here makes the compiler hoist the whole matrix stream):
+// keep[t] = 0 zeroes level t (Dirichlet), proj[t] = 1 projects it tangentially
+// with the corner normals nrm (free-slip).
 template <typename T, typename S, bool DIAG>
 __device__ __forceinline__ void stored_cell_sums(const StoredArgs<T, S>& a, int64_t cell, const int64_t latc[4],
-                                                 int cc, T scale, T yb[4][3], T yt[4][3])
+                                                 int cc, T scale, const T keep[2], const T proj[2],
+                                                 const T nrm[4][3], T yb[4][3], T yt[4][3])
 {
     constexpr int WCORN[2][3] = {{0, 1, 2}, {3, 2, 1}};
     const int64_t N = 10LL * a.n * a.n * a.nr;
@@ -338,6 +340,16 @@ __device__ __forceinline__ void stored_cell_sums(const StoredArgs<T, S>& a, int6
             const int64_t ib = latc[WCORN[w][j % 3]] * a.nr + cc + (j >= 3);
 #pragma unroll
             for (int k = 0; k < 3; ++k) xs[k * 6 + j] = scale * a.src[k * N + ib];
+        }
+        if constexpr (!DIAG) {
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                const int t = j >= 3;
+                const T* nn = nrm[WCORN[w][j % 3]];
+                const T un = proj[t] * (nn[0] * xs[j] + nn[1] * xs[6 + j] + nn[2] * xs[12 + j]);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) xs[k * 6 + j] = keep[t] * (xs[k * 6 + j] - un * nn[k]);
+            }
         }
         // dof i of the wedge (comp*6 + node) accumulates into its corner sum
         auto Y = [&](int i) -> T& {
@@ -385,12 +397,25 @@ __global__ __launch_bounds__(256) void k_stored_cells(StoredArgs<T, S> a)
     const int c = 64 * h + lane;
     const bool valid = c < ncr;
     const int cc = valid ? c : ncr - 1;
-    const bool boundary_layer = (c == 0 && a.bc_cmb != TN_NEUMANN) || (c == ncr - 1 && a.bc_surf != TN_NEUMANN);
-    const T scale = (valid && !boundary_layer) ? T(1) : T(0);
+    const T scale = valid ? T(1) : T(0);
+    // trial side of the boundary levels: bottom of cell 0, top of cell ncr - 1
+    const int mb = c == 0 ? a.bc_cmb : TN_NEUMANN, mt = c == ncr - 1 ? a.bc_surf : TN_NEUMANN;
+    const T keep[2] = {mb == TN_DIRICHLET ? T(0) : T(1), mt == TN_DIRICHLET ? T(0) : T(1)};
+    const T proj[2] = {mb == TN_FREESLIP ? T(1) : T(0), mt == TN_FREESLIP ? T(1) : T(0)};
     const int64_t ncell = ncol * ncr;
     const int toplane = min(63, ncr - 1 - 64 * h);
     const int64_t lat00 = ((int64_t)d * a.n + x) * a.n + y;
     const int64_t latc[4] = {lat00, lat00 + a.n, lat00 + 1, lat00 + a.n + 1};
+    T nrm[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if (a.bc_cmb == TN_FREESLIP || a.bc_surf == TN_FREESLIP) {  // wave-uniform
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double* cp = a.coords + 3 * latc[k];
+            const double inv = 1.0 / sqrt(cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2]);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) nrm[k][j] = (T)(cp[j] * inv);
+        }
+    }
     T yb[4][3], yt[4][3];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -398,7 +423,7 @@ __global__ __launch_bounds__(256) void k_stored_cells(StoredArgs<T, S> a)
         for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
     // (the lane condition also keeps the compiler from hoisting the whole
     // matrix stream of both wedges into registers ahead of the arithmetic)
-    if (scale != T(0)) stored_cell_sums<T, S, DIAG>(a, col * ncr + cc, latc, cc, scale, yb, yt);
+    if (valid) stored_cell_sums<T, S, DIAG>(a, col * ncr + cc, latc, cc, scale, keep, proj, nrm, yb, yt);
     // radial merge (level c = bottom of cell c + top of cell c-1) and write-out
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -411,12 +436,18 @@ __global__ __launch_bounds__(256) void k_stored_cells(StoredArgs<T, S> a)
         }
 }
 
-// Node copy (d, X, Y, level): sum of its lateral quads' corner values.
-template <typename T>
-__global__ __launch_bounds__(256) void k_stored_gather(const T* __restrict__ csum, const T* __restrict__ ctop,
-                                                       T* __restrict__ dst, int n, int nr, int add)
+// Node copy (d, X, Y, level): sum of its lateral quads' corner values, then on
+// Dirichlet / free-slip shells the test side (boundary-node forms blk):
+//   Dirichlet  y = diag(B) x            (diagonal mode: already diag(A) x)
+//   free-slip  y = P y + (n^T B n)(n . x) n
+//              diagonal mode: R^T diag(R B R^T) R x, rows of R = (n, t1, t2)
+// and the fs/fs rotation penalty gamma x p (FP64) -- the same algebraic
+// boundary treatment as the matrix-free apply; stores (Replace) or adds (Add).
+template <typename T, typename S, bool DIAG>
+__global__ __launch_bounds__(256) void k_stored_gather(StoredArgs<T, S> a)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = a.n, nr = a.nr;
     const int64_t N = 10LL * n * n * nr;
     if (tid >= N) return;
     const int ncr = nr - 1, ncl = n - 1;
@@ -436,143 +467,156 @@ __global__ __launch_bounds__(256) void k_stored_gather(const T* __restrict__ csu
         const int64_t col = ((int64_t)d * ncl + qx) * ncl + qy;
         if (lev < ncr) {
 #pragma unroll
-            for (int j = 0; j < 3; ++j) v[j] += csum[(int64_t)(k * 3 + j) * ncell + col * ncr + lev];
+            for (int j = 0; j < 3; ++j) v[j] += a.csum[(int64_t)(k * 3 + j) * ncell + col * ncr + lev];
         }
         // first level of a chunk > 0, or the top level: the previous chunk's top
         if ((lev > 0 && (lev & 63) == 0) || lev == ncr) {
             const int hp = (lev - 1) >> 6;
 #pragma unroll
-            for (int j = 0; j < 3; ++j) v[j] += ctop[(int64_t)(k * 3 + j) * ncol * nch + col * nch + hp];
+            for (int j = 0; j < 3; ++j) v[j] += a.ctop[(int64_t)(k * 3 + j) * ncol * nch + col * nch + hp];
         }
     }
-    const int64_t idx = tid;
+    const int mode = lev == 0 ? a.bc_cmb : (lev == ncr ? a.bc_surf : TN_NEUMANN);
+    if (mode != TN_NEUMANN) {
+        const T* bk = a.blk + ((lev == 0 ? 0 : 10LL * n * n) + lat) * 3;
+        const T x[3] = {a.src[tid], a.src[N + tid], a.src[2 * N + tid]};
+        if (mode == TN_DIRICHLET) {
+            if constexpr (!DIAG) {
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        if (add)
-            dst[j * N + idx] += v[j];
-        else
-            dst[j * N + idx] = v[j];
-    }
-}
-
-// Boundary cell layers: one thread per (layer, lateral cell); atomics into dst.
-template <typename T, typename S, bool DIAG>
-__global__ __launch_bounds__(128) void k_stored_boundary(StoredArgs<T, S> a)
-{
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ncr = a.nr - 1, ncl = a.n - 1;
-    const int64_t nlat = 10LL * ncl * ncl;
-    if (tid >= 2 * nlat) return;
-    const int layer = (int)(tid / nlat);  // 0: CMB cell layer, 1: surface cell layer
-    const int rc = layer == 0 ? 0 : ncr - 1;
-    if ((layer == 0 ? a.bc_cmb : a.bc_surf) == TN_NEUMANN) return;
-    int64_t lat = tid % nlat;
-    const int y = (int)(lat % ncl);
-    lat /= ncl;
-    const int x = (int)(lat % ncl);
-    const int d = (int)(lat / ncl);
-    const int64_t ncell = 10LL * ncl * ncl * ncr;
-    const int64_t cell = (((int64_t)d * ncl + x) * ncl + y) * ncr + rc;
-    const int64_t N = 10LL * a.n * a.n * a.nr;
-    const int64_t lat00 = ((int64_t)d * a.n + x) * a.n + y;
-    const int64_t latc[4] = {lat00, lat00 + a.n, lat00 + 1, lat00 + a.n + 1};
-    constexpr int WCORN[2][3] = {{0, 1, 2}, {3, 2, 1}};
-    // a single-layer shell (ncr == 1) has both shells on this cell
-    const int bmode = rc == 0 ? a.bc_cmb : TN_NEUMANN;
-    const int tmode = rc == ncr - 1 ? a.bc_surf : TN_NEUMANN;
-    if (layer == 1 && rc == 0) return;  // same cell as layer 0
-    double nrm[4][3];
-    for (int c = 0; c < 4; ++c) {
-        const double* cp = a.coords + 3 * latc[c];
-        const double inv = 1.0 / sqrt(cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2]);
-        for (int k = 0; k < 3; ++k) nrm[c][k] = cp[k] * inv;
-    }
-    for (int w = 0; w < 2; ++w) {
-        T xs[18];  // dimension-wise: comp*6 + node
-        for (int j = 0; j < 6; ++j) {
-            const int c = WCORN[w][j % 3];
-            const int64_t ib = latc[c] * a.nr + rc + (j >= 3);
-            for (int k = 0; k < 3; ++k) xs[k * 6 + j] = a.src[k * N + ib];
-        }
-        auto A = [&](int i, int j) -> T { return (T)a.stored[stored_at<S>(cell, w, pk(i, j))]; };
-        T ys[18];
-        if constexpr (DIAG) {
-            for (int i = 0; i < 18; ++i) ys[i] = A(i, i) * xs[i];
-            // free-slip boundary nodes: R^T diag(R A_bb R^T) R x
-            for (int j = 0; j < 6; ++j) {
-                const int mode = j < 3 ? bmode : tmode;
-                if (mode != TN_FREESLIP) continue;
-                const int c = WCORN[w][j % 3];
-                T Ab[3][3];
-                for (int p = 0; p < 3; ++p)
-                    for (int q2 = 0; q2 < 3; ++q2) {
-                        const int r = p * 6 + j, cq = q2 * 6 + j;
-                        Ab[p][q2] = r <= cq ? A(r, cq) : A(cq, r);
-                    }
-                double n3[3] = {nrm[c][0], nrm[c][1], nrm[c][2]}, t1[3], t2[3];
+                for (int j = 0; j < 3; ++j) v[j] = bk[j] * x[j];
+            }
+        } else {
+            const double* cp = a.coords + 3 * lat;
+            const double inv = 1.0 / sqrt(cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2]);
+            double n3[3] = {cp[0] * inv, cp[1] * inv, cp[2] * inv};
+            if constexpr (DIAG) {
+                double t1[3], t2[3];
                 nt_basis_d(n3, t1, t2);
                 const double* rows[3] = {n3, t1, t2};
                 double out[3] = {0, 0, 0};
                 for (int b = 0; b < 3; ++b) {
                     const double* e = rows[b];
-                    double Ae[3];
-                    for (int p = 0; p < 3; ++p) Ae[p] = Ab[p][0] * e[0] + Ab[p][1] * e[1] + Ab[p][2] * e[2];
-                    const double dgl = e[0] * Ae[0] + e[1] * Ae[1] + e[2] * Ae[2];
-                    const double ex = e[0] * xs[j] + e[1] * xs[6 + j] + e[2] * xs[12 + j];
-                    for (int p = 0; p < 3; ++p) out[p] += dgl * ex * e[p];
+                    const double ex = (double)bk[b] * (e[0] * x[0] + e[1] * x[1] + e[2] * x[2]);
+                    for (int j = 0; j < 3; ++j) out[j] += ex * e[j];
                 }
-                for (int p = 0; p < 3; ++p) ys[p * 6 + j] = (T)out[p];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) v[j] = (T)out[j];
+            } else {
+                const T nx = (T)n3[0], ny = (T)n3[1], nz = (T)n3[2];
+                const T corr = bk[0] * (nx * x[0] + ny * x[1] + nz * x[2]) - (nx * v[0] + ny * v[1] + nz * v[2]);
+                v[0] += corr * nx;
+                v[1] += corr * ny;
+                v[2] += corr * nz;
             }
-        } else {
-            T xm[18];
-            for (int i = 0; i < 18; ++i) xm[i] = xs[i];
-            // trial side
-            for (int j = 0; j < 6; ++j) {
-                const int mode = j < 3 ? bmode : tmode;
-                if (mode == TN_DIRICHLET) {
-                    xm[j] = xm[6 + j] = xm[12 + j] = T(0);
-                } else if (mode == TN_FREESLIP) {
-                    const int c = WCORN[w][j % 3];
-                    const T un = (T)nrm[c][0] * xm[j] + (T)nrm[c][1] * xm[6 + j] + (T)nrm[c][2] * xm[12 + j];
-                    for (int k = 0; k < 3; ++k) xm[k * 6 + j] -= un * (T)nrm[c][k];
-                }
-            }
-            for (int i = 0; i < 18; ++i) ys[i] = T(0);
-            for (int i = 0; i < 18; ++i)
-                for (int j = i; j < 18; ++j) {
-                    const T aij = A(i, j);
-                    ys[i] += aij * xm[j];
-                    if (j != i) ys[j] += aij * xm[i];
-                }
-            // test side
-            for (int j = 0; j < 6; ++j) {
-                const int mode = j < 3 ? bmode : tmode;
-                if (mode == TN_DIRICHLET) {
-                    for (int k = 0; k < 3; ++k) ys[k * 6 + j] = A(k * 6 + j, k * 6 + j) * xs[k * 6 + j];
-                } else if (mode == TN_FREESLIP) {
-                    const int c = WCORN[w][j % 3];
-                    T nA[3];  // A_bb n
-                    for (int p = 0; p < 3; ++p) {
-                        T sacc = 0;
-                        for (int q2 = 0; q2 < 3; ++q2) {
-                            const int r = p * 6 + j, cq = q2 * 6 + j;
-                            sacc += (r <= cq ? A(r, cq) : A(cq, r)) * (T)nrm[c][q2];
-                        }
-                        nA[p] = sacc;
-                    }
-                    const T ann = (T)nrm[c][0] * nA[0] + (T)nrm[c][1] * nA[1] + (T)nrm[c][2] * nA[2];
-                    const T un = (T)nrm[c][0] * xs[j] + (T)nrm[c][1] * xs[6 + j] + (T)nrm[c][2] * xs[12 + j];
-                    const T yn = (T)nrm[c][0] * ys[j] + (T)nrm[c][1] * ys[6 + j] + (T)nrm[c][2] * ys[12 + j];
-                    for (int k = 0; k < 3; ++k) ys[k * 6 + j] += (ann * un - yn) * (T)nrm[c][k];
-                }
-            }
-        }
-        for (int j = 0; j < 6; ++j) {
-            const int c = WCORN[w][j % 3];
-            const int64_t ib = latc[c] * a.nr + rc + (j >= 3);
-            for (int k = 0; k < 3; ++k) unsafeAtomicAdd(a.dst + k * N + ib, ys[k * 6 + j]);
         }
     }
+    if (a.pen) {
+        const double* bt = a.pen;
+        const double* Sm = a.pen + 4;
+        const double g0 = Sm[0] * bt[0] + Sm[1] * bt[1] + Sm[2] * bt[2];
+        const double g1 = Sm[3] * bt[0] + Sm[4] * bt[1] + Sm[5] * bt[2];
+        const double g2 = Sm[6] * bt[0] + Sm[7] * bt[1] + Sm[8] * bt[2];
+        const double r = a.radii[lev];
+        const double px = a.coords[3 * lat] * r, py = a.coords[3 * lat + 1] * r, pz = a.coords[3 * lat + 2] * r;
+        v[0] += (T)(g1 * pz - g2 * py);
+        v[1] += (T)(g2 * px - g0 * pz);
+        v[2] += (T)(g0 * py - g1 * px);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        if (a.add)
+            a.dst[j * N + tid] += v[j];
+        else
+            a.dst[j * N + tid] = v[j];
+    }
+}
+
+// Boundary-node forms of the stored operator: one thread per (shell, lateral
+// node); B sums the node's 3x3 diagonal blocks over the boundary-layer
+// wedges of its diamond that contain it.
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_stored_blk(StoredArgs<T, S> a, T* __restrict__ blk)
+{
+    constexpr int WCORN[2][3] = {{0, 1, 2}, {3, 2, 1}};
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = a.n, ncl = n - 1, ncr = a.nr - 1;
+    const int64_t nlat = 10LL * n * n;
+    if (tid >= 2 * nlat) return;
+    const int shell = (int)(tid / nlat);
+    const int mode = shell == 0 ? a.bc_cmb : a.bc_surf;
+    if (mode == TN_NEUMANN) return;
+    const int64_t lat = tid % nlat;
+    const int Yn = (int)(lat % n), Xn = (int)((lat / n) % n), d = (int)(lat / ((int64_t)n * n));
+    const int rc = shell == 0 ? 0 : ncr - 1;
+    double B[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < 4; ++k) {
+        const int qx = Xn - (k & 1), qy = Yn - (k >> 1);
+        if (qx < 0 || qy < 0 || qx > ncl - 1 || qy > ncl - 1) continue;
+        const int64_t cell = (((int64_t)d * ncl + qx) * ncl + qy) * ncr + rc;
+        for (int w = 0; w < 2; ++w) {
+            int m = -1;
+            for (int i = 0; i < 3; ++i)
+                if (WCORN[w][i] == k) m = i;
+            if (m < 0) continue;
+            const int j = m + (shell == 0 ? 0 : 3);
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) {
+                    const int r = p * 6 + j, cq = q * 6 + j;
+                    B[p][q] += (double)a.stored[stored_at<S>(cell, w, r <= cq ? pk(r, cq) : pk(cq, r))];
+                }
+        }
+    }
+    double e[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    if (mode == TN_FREESLIP) {
+        const double* cp = a.coords + 3 * lat;
+        const double inv = 1.0 / sqrt(cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2]);
+        for (int i = 0; i < 3; ++i) e[0][i] = cp[i] * inv;
+        nt_basis_d(e[0], e[1], e[2]);
+    }
+    for (int b = 0; b < 3; ++b) {
+        double q = 0.0;
+        for (int p = 0; p < 3; ++p)
+            for (int r = 0; r < 3; ++r) q += e[b][p] * B[p][r] * e[b][r];
+        blk[(shell * nlat + lat) * 3 + b] = (T)q;
+    }
+}
+
+// Jacobi scaling with 3x3 blocks on free-slip shells: z = dinv * z entrywise,
+// except at free-slip boundary node copies, where z = R^T diag(1 / d) R z with
+// rows of R = (n, t1, t2) and d = the node's forms (n^T B n, t1^T B t1,
+// t2^T B t2) -- the inverse of the operator's free-slip diagonal
+// R^T diag(R B R^T) R, which entrywise inversion of that diagonal is not.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fs_block_scale(T* __restrict__ z, const T* __restrict__ dinv,
+                                                        const T* __restrict__ blk, const double* __restrict__ coords,
+                                                        int n, int nr, int bc_cmb, int bc_surf)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = 10LL * n * n * nr;
+    if (tid >= N) return;
+    const int lev = (int)(tid % nr);
+    const int mode = lev == 0 ? bc_cmb : (lev == nr - 1 ? bc_surf : TN_NEUMANN);
+    if (mode != TN_FREESLIP) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) z[j * N + tid] *= dinv[j * N + tid];
+        return;
+    }
+    const int64_t lat = tid / nr;
+    const T* bk = blk + ((lev == 0 ? 0 : 10LL * n * n) + lat) * 3;
+    const double* cp = coords + 3 * lat;
+    const double inv = 1.0 / sqrt(cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2]);
+    double n3[3] = {cp[0] * inv, cp[1] * inv, cp[2] * inv}, t1[3], t2[3];
+    nt_basis_d(n3, t1, t2);
+    const double* rows[3] = {n3, t1, t2};
+    const double r[3] = {(double)z[tid], (double)z[N + tid], (double)z[2 * N + tid]};
+    double out[3] = {0, 0, 0};
+    for (int b = 0; b < 3; ++b) {
+        const double* e = rows[b];
+        const double c = (e[0] * r[0] + e[1] * r[1] + e[2] * r[2]) / (double)bk[b];
+        for (int j = 0; j < 3; ++j) out[j] += c * e[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) z[j * N + tid] = (T)out[j];
 }
 
 }  // namespace
@@ -613,28 +657,44 @@ void launch_stored_apply(const StoredArgs<T, S>& a, bool diagonal, hipStream_t s
 {
     TN_REQUIRE(a.nr >= 3, "stored apply: at least two radial cell layers");
     TN_REQUIRE(a.csum && a.ctop, "stored apply: no corner-sum buffers");
+    TN_REQUIRE(a.blk || (a.bc_cmb == TN_NEUMANN && a.bc_surf == TN_NEUMANN), "stored apply: no boundary forms");
     const int ncr = a.nr - 1, ncl = a.n - 1, nch = (ncr + 63) / 64;
     const int64_t waves = 10LL * ncl * ncl * nch;
     const dim3 gr((unsigned)((waves + 3) / 4)), bl(256);
-    if (diagonal)
-        hipLaunchKernelGGL((k_stored_cells<T, S, true>), gr, bl, 0, s, a);
-    else
-        hipLaunchKernelGGL((k_stored_cells<T, S, false>), gr, bl, 0, s, a);
-    TN_HIP(hipGetLastError());
     const int64_t N = 10LL * a.n * a.n * a.nr;
-    hipLaunchKernelGGL(k_stored_gather<T>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, (const T*)a.csum,
-                       (const T*)a.ctop, a.dst, a.n, a.nr, a.add);
-    TN_HIP(hipGetLastError());
-    if (a.bc_cmb != TN_NEUMANN || a.bc_surf != TN_NEUMANN) {
-        const int64_t total = 2LL * 10 * (a.n - 1) * (a.n - 1);
-        const dim3 gb((unsigned)((total + 127) / 128)), bb(128);
-        if (diagonal)
-            hipLaunchKernelGGL((k_stored_boundary<T, S, true>), gb, bb, 0, s, a);
-        else
-            hipLaunchKernelGGL((k_stored_boundary<T, S, false>), gb, bb, 0, s, a);
+    const dim3 gg((unsigned)((N + 255) / 256)), bg(256);
+    if (diagonal) {
+        hipLaunchKernelGGL((k_stored_cells<T, S, true>), gr, bl, 0, s, a);
         TN_HIP(hipGetLastError());
+        hipLaunchKernelGGL((k_stored_gather<T, S, true>), gg, bg, 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_stored_cells<T, S, false>), gr, bl, 0, s, a);
+        TN_HIP(hipGetLastError());
+        hipLaunchKernelGGL((k_stored_gather<T, S, false>), gg, bg, 0, s, a);
     }
+    TN_HIP(hipGetLastError());
 }
+
+template <typename T, typename S>
+void launch_stored_blk(const StoredArgs<T, S>& a, T* blk, hipStream_t s)
+{
+    const int64_t total = 2LL * 10 * a.n * a.n;
+    hipLaunchKernelGGL((k_stored_blk<T, S>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, blk);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_fs_block_scale(const MeshLevel& m, T* z, const T* dinv, const T* blk, int bc_cmb, int bc_surf,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fs_block_scale<T>, dim3((unsigned)((m.N + 255) / 256)), dim3(256), 0, s, z, dinv, blk,
+                       m.d_coords.p, m.n, m.nr, bc_cmb, bc_surf);
+    TN_HIP(hipGetLastError());
+}
+template void launch_fs_block_scale<double>(const MeshLevel&, double*, const double*, const double*, int, int,
+                                            hipStream_t);
+template void launch_fs_block_scale<float>(const MeshLevel&, float*, const float*, const float*, int, int,
+                                           hipStream_t);
 
 // elements of the corner-sum buffers: 12 per cell + 12 per (quad, chunk)
 int64_t stored_cellsum_elems(int n, int nr)
@@ -657,7 +717,9 @@ TN_INST_GCA(float, double, double)
 TN_INST_GCA(float, double, float)
 TN_INST_GCA(float, float, double)
 TN_INST_GCA(float, float, float)
-#define TN_INST_ST(T, S) template void launch_stored_apply<T, S>(const StoredArgs<T, S>&, bool, hipStream_t);
+#define TN_INST_ST(T, S)                                                                   \
+    template void launch_stored_apply<T, S>(const StoredArgs<T, S>&, bool, hipStream_t); \
+    template void launch_stored_blk<T, S>(const StoredArgs<T, S>&, T*, hipStream_t);
 TN_INST_ST(double, double)
 TN_INST_ST(double, float)
 TN_INST_ST(float, float)

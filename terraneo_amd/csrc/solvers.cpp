@@ -239,15 +239,23 @@ Chebyshev::Chebyshev(int order, int iterations, BVec inv_diag, BVec t0, BVec t1,
     TN_REQUIRE(order >= 1, "Chebyshev order must be at least 1");
 }
 
+void Chebyshev::apply_scale(Ctx& c, const BVec& z)
+{
+    if (scale)
+        scale(c, z);
+    else
+        v_mul(c, z, dinv_);
+}
+
 namespace {
 struct DiagScaled : LinOp {
     LinOp& A;
-    BVec dinv;
-    DiagScaled(LinOp& a, BVec d) : A(a), dinv(d) {}
+    std::function<void(Ctx&, const BVec&)> sc;
+    DiagScaled(LinOp& a, std::function<void(Ctx&, const BVec&)> s) : A(a), sc(std::move(s)) {}
     void apply(Ctx& c, const BVec& x, const BVec& y) override
     {
         A.apply(c, x, y);
-        v_mul(c, y, dinv);
+        sc(c, y);
     }
 };
 }  // namespace
@@ -256,7 +264,7 @@ void Chebyshev::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
 {
     if (need_lambda_) {
         // power iteration on D^-1 A from a deterministic random start
-        DiagScaled op(A, dinv_);
+        DiagScaled op(A, [this](Ctx& cc, const BVec& z) { apply_scale(cc, z); });
         BVec& it = d_;
         BVec& aux = z_;
         v_randomize(c, it, seed_);
@@ -280,7 +288,7 @@ void Chebyshev::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
     for (int it = 0; it < iters_; ++it) {
         A.apply(c, x, z_);
         v_lincomb(c, z_, {1.0, -1.0}, {&b, &z_});
-        v_mul(c, z_, dinv_);
+        apply_scale(c, z_);
         v_lincomb(c, d_, {1.0 / theta}, {&z_});
         v_lincomb(c, x, {1.0, 1.0}, {&x, &d_});
         for (int i = 2; i <= order_; ++i) {
@@ -289,7 +297,7 @@ void Chebyshev::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
             const double alpha = 2.0 * rho_new / delta;
             A.apply(c, x, z_);
             v_lincomb(c, z_, {1.0, -1.0}, {&b, &z_});
-            v_mul(c, z_, dinv_);
+            apply_scale(c, z_);
             v_lincomb(c, d_, {alpha, beta}, {&z_, &d_});
             v_lincomb(c, x, {1.0, 1.0}, {&x, &d_});
             rho = rho_new;

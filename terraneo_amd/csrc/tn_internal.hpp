@@ -160,6 +160,9 @@ template <typename TD, typename TS>
 void launch_cast(TD* y, const TS* x, int64_t count, hipStream_t s);
 template <typename T>
 void launch_randomize(T* y, int64_t count, uint64_t seed, hipStream_t s);
+// *out_dev = max |y[i]| (out_dev must hold 0.0 on entry)
+template <typename T>
+void launch_max_abs(const T* y, int64_t count, double* out_dev, hipStream_t s);
 template <typename T>
 void launch_nonfinite(const T* y, int64_t count, int* flag_dev, hipStream_t s);
 template <typename T>

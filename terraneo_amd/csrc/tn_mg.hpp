@@ -47,7 +47,17 @@ struct StoredArgs {
     int add;     // 1: dst += A x, 0: dst = A x
     T* csum;     // [12][cells] merged corner sums (pass 1 -> pass 2)
     T* ctop;     // [12][10 (n-1)^2 chunks] chunk-top level sums
+    const T* blk;        // [2][10 n n][3] boundary-node quadratic forms (launch_stored_blk), or null
+    const double* pen;   // fs/fs penalty: [0..3) beta, [4..13) S, or null
+    const double* radii; // [nr] (penalty positions)
 };
+
+// boundary-node quadratic forms e^T B e of the stored operator (B: the 3x3
+// diagonal block at the node copy, summed over its diamond's boundary-layer
+// wedges; e = axes for Dirichlet, n / t1 / t2 for free-slip), layout of the
+// matrix-free operator's forms
+template <typename T, typename S>
+void launch_stored_blk(const StoredArgs<T, S>& a, T* blk, hipStream_t s);
 
 template <typename T, typename S, typename SF>
 struct GcaArgs {
@@ -97,5 +107,9 @@ template <typename T, typename S>
 void launch_stored_apply(const StoredArgs<T, S>& a, bool diagonal, hipStream_t s);
 // elements of StoredArgs::csum + ctop for an n x n x nr mesh
 int64_t stored_cellsum_elems(int n, int nr);
+// z = dinv * z, with 3x3 block inverses R^T diag(1/blk) R on free-slip shells
+template <typename T>
+void launch_fs_block_scale(const MeshLevel& m, T* z, const T* dinv, const T* blk, int bc_cmb, int bc_surf,
+                           hipStream_t s);
 
 }  // namespace tn

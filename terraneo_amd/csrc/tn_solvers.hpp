@@ -118,8 +118,11 @@ class Chebyshev : public Solver {
         lambda_ = l;
         need_lambda_ = false;
     }
+    // z <- D^-1 z; default: entrywise with the inverse diagonal (the reference's)
+    std::function<void(Ctx&, const BVec&)> scale;
 
   private:
+    void apply_scale(Ctx& c, const BVec& z);
     int order_, iters_, power_its_;
     uint64_t seed_;
     BVec dinv_, d_, z_;

@@ -123,6 +123,38 @@ def test_blas_dot_and_lincomb(cuda):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_blas_vector_ops(cuda, dt):
+    """invert_entries, scale_with_vector, max_abs_entry, has_nan_or_inf,
+    randomize (grid_operations.hpp) against torch."""
+    import torch
+
+    m, om, eta, x = _setup(3, 2, seed=5)
+    tdt = torch.float64 if dt == "f64" else torch.float32
+    xd = torch.from_numpy(x).to(cuda).to(tdt) + 2.0  # away from zero
+    d = torch.from_numpy(consistent_random(om, 3, 8).reshape(-1)).to(cuda).to(tdt)
+    y = xd.clone()
+    m.invert_entries(y)
+    assert torch.equal(y, 1.0 / xd)
+    y = xd.clone()
+    m.scale_with_vector(y, d)
+    assert torch.equal(y, xd * d)
+    assert m.max_abs_entry(d) == float(d.abs().max())
+    assert not m.has_nan_or_inf(d)
+    d2 = d.clone()
+    k = d2.numel() // 2
+    d2[k] = float("inf")
+    assert m.has_nan_or_inf(d2)
+    d2[k] = float("nan")
+    assert m.has_nan_or_inf(d2)
+    r1, r2 = torch.empty_like(d), torch.empty_like(d)
+    m.randomize(r1, 7)
+    m.randomize(r2, 7)
+    assert torch.equal(r1, r2) and float(r1.min()) >= 0.0 and float(r1.max()) < 1.0
+    assert abs(float(r1.double().mean()) - 0.5) < 0.01
+
+
+@pytest.mark.gpu
 def test_eps_viscosity_update_in_place(cuda):
     """Derived boundary data follows the viscosity after set_eta (same buffer)."""
     import torch
