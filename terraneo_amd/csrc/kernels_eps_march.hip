@@ -123,8 +123,7 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
         const T r0 = a.radii[cc], r1 = a.radii[cc + 1];
         const T half_dr = T(0.5) * (r1 - r0), r_mid = T(0.5) * (r0 + r1);
         const T alpha = T(1) / r_mid, beta = T(1) / half_dr;
-        // lanes past the last radial cell compute on clamped data with zero weight
-        const T kr = valid ? r_mid * r_mid * half_dr * T(1.0 / 6.0) : T(0);
+        const T kr = r_mid * r_mid * half_dr * T(1.0 / 6.0);
         const int blev = c == 0 ? bmode : TN_NEUMANN;
         const int tlev = c == ncr - 1 ? tmode : TN_NEUMANN;
         T c01[3] = {0, 0, 0}, c11[3] = {0, 0, 0}, c01t[3] = {0, 0, 0}, c11t[3] = {0, 0, 0};
@@ -134,19 +133,19 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
             T(*E)[65] = ebuf[step & 1][w];
             const int64_t lat00 = ((int64_t)d * n + x) * n + qy;
             T yb[4][3], yt[4][3];
-            if (!quad_ok) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k)
+            for (int k = 0; k < 4; ++k)
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
-            } else {
+                for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
+            if (quad_ok) {
                 // wave-uniform: scalar loads into SGPRs
                 const int goff = __builtin_amdgcn_readfirstlane(((d * ncl + x) * ncl + qy) * 2 * GEO_STRIDE);
                 const T* gsrc = geo_tab + goff;
                 T geo[2 * GEO_STRIDE];
 #pragma unroll
                 for (int i = 0; i < 2 * GEO_STRIDE; ++i) geo[i] = gsrc[i];
-                {
+                // (the lane condition also keeps the compiler's load hoisting in check)
+                if (valid) {
                     const int64_t latc[4] = {lat00, lat00 + n, lat00 + 1, lat00 + n + 1};
                     const T ecell = a.eta_cell ? a.eta_cell[(((int64_t)d * ncl + x) * ncl + qy) * ncr + cc] : T(0);
                     if constexpr (DIAG) {

@@ -1,0 +1,84 @@
+"""Per-operator timing on one GPU, for profiling the kernels one at a time.
+
+    python -m terraneo_amd.kernel_timing stored [--lat 7] [--reps 10]
+        the level lat-1 Galerkin (stored-matrix) operator of a 3-level
+        free-slip hierarchy (eta = 1), FP64 and FP32 storage, plain and
+        diagonal mode
+    python -m terraneo_amd.kernel_timing coupling [--lat 7] [--reps 20]
+        B, B^T (Replace and Add) and the full Stokes block apply, velocity
+        lateral level lat x (2^lat + 1) radial nodes, free-slip shells
+
+Each line reports the mean wall time per call over `reps` calls after three
+warm-up calls (stream-synchronised), so that a `rocprofv3 --kernel-trace
+--stats` run of the same command attributes the time to kernels.
+"""
+import argparse
+import time
+
+import torch
+
+from . import api
+from ._capi import TN_ADD, TN_FREESLIP
+from .problem import StokesProblem
+
+
+def _time(fn, reps):
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
+def stored(lat, reps):
+    dev = torch.device("cuda:0")
+    for prec in ("f64", "f32"):
+        pr = StokesProblem(lat, lat, 3, bc=(TN_FREESLIP, TN_FREESLIP), device=dev, prec_dtype=prec,
+                           coarsest_f64=True, gmg=api.gmg_options(coarse_solver="pcg"))
+        pr.set_viscosity(eta=torch.ones(pr.vel.N, dtype=torch.float64, device=dev))
+        op, m = pr.ops[-2], pr.meshes[-2]
+        dt = torch.float64 if prec == "f64" else torch.float32
+        x = torch.randn(3 * m.N, dtype=dt, device=dev)
+        y = torch.empty_like(x)
+        ncell = 10 * (m.n - 1) ** 2 * (m.nr - 1)
+        gb = ncell * 342 * (8 if prec == "f64" else 4) / 1e9
+        for diag in (False, True):
+            op.set_diagonal(diag)
+            ms = _time(lambda: op.apply(x, y), reps)
+            print(f"level {lat - 1} stored {prec} diagonal={diag}: {ms:.3f} ms per apply "
+                  f"(matrix stream {gb:.2f} GB)", flush=True)
+        op.set_diagonal(False)
+        del pr
+        torch.cuda.empty_cache()
+
+
+def coupling(lat, reps):
+    dev = torch.device("cuda:0")
+    vel = api.Mesh(lat, lat, 0.55, 1.0)
+    pre = api.Mesh(lat - 1, lat - 1, 0.55, 1.0)
+    eta = torch.ones(vel.N, dtype=torch.float64, device=dev)
+    A = api.Eps(vel, eta, TN_FREESLIP, TN_FREESLIP)
+    K = api.Stokes(vel, pre, A)
+    u = torch.randn(3 * vel.N, dtype=torch.float64, device=dev)
+    p = torch.randn(pre.N, dtype=torch.float64, device=dev)
+    yu, yp = torch.empty_like(u), torch.empty_like(p)
+    for name, fn in [("B (div)", lambda: K.div(u, yp)), ("B^T (grad, Replace)", lambda: K.grad(p, yu)),
+                     ("B^T (grad, Add)", lambda: K.grad(p, yu, apply_mode=TN_ADD)),
+                     ("Stokes block", lambda: K.apply(u, p, yu, yp))]:
+        print(f"velocity level {lat}: {name}: {_time(fn, reps):.3f} ms per call", flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["stored", "coupling"])
+    ap.add_argument("--lat", type=int, default=7)
+    ap.add_argument("--reps", type=int, default=10)
+    a = ap.parse_args()
+    (stored if a.what == "stored" else coupling)(a.lat, a.reps)
+
+
+if __name__ == "__main__":
+    main()
