@@ -78,6 +78,22 @@ __global__ __launch_bounds__(256) void k_dot_owned(const T* __restrict__ a, cons
     block_accumulate(s, out);
 }
 
+// One Chebyshev step fused (chebyshev.hpp:100-168 does it in four passes):
+// z = dinv (b - Az); d = alpha z + beta d; x += d.  az holds A x on entry and
+// is not written.  FIRST: beta = 0 and d is not read (it may hold anything).
+template <typename T, bool FIRST>
+__global__ __launch_bounds__(256) void k_cheby_update(T* __restrict__ x, T* __restrict__ d, const T* __restrict__ az,
+                                                      const T* __restrict__ b, const T* __restrict__ dinv,
+                                                      int64_t count, T alpha, T beta)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        const T z = dinv[i] * (b[i] - az[i]);
+        const T dn = FIRST ? alpha * z : alpha * z + beta * d[i];
+        d[i] = dn;
+        x[i] += dn;
+    }
+}
+
 // y[i] -= scale * sum[0] (sum: a device scalar from a preceding reduction)
 template <typename T>
 __global__ __launch_bounds__(256) void k_sub_scaled(T* __restrict__ y, int64_t count, const double* __restrict__ sum,
@@ -290,6 +306,18 @@ void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, dou
 }
 
 template <typename T>
+void launch_cheby_update(T* x, T* d, const T* az, const T* b, const T* dinv, int64_t count, double alpha, double beta,
+                         bool first, hipStream_t s)
+{
+    const dim3 gr(grid_for(count)), bl(256);
+    if (first)
+        hipLaunchKernelGGL((k_cheby_update<T, true>), gr, bl, 0, s, x, d, az, b, dinv, count, (T)alpha, (T)beta);
+    else
+        hipLaunchKernelGGL((k_cheby_update<T, false>), gr, bl, 0, s, x, d, az, b, dinv, count, (T)alpha, (T)beta);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
 void launch_sub_scaled(T* y, int64_t count, const double* sum_dev, double scale, hipStream_t s)
 {
     hipLaunchKernelGGL(k_sub_scaled<T>, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, y, count, sum_dev, scale);
@@ -391,6 +419,8 @@ template void launch_cast<float, double>(float*, const double*, int64_t, hipStre
     template void launch_lincomb<T>(int64_t, T*, int, const double*, const T* const*, double, hipStream_t);     \
     template void launch_dot_owned<T>(const MeshLevel&, int, const T*, const T*, double*, hipStream_t);          \
     template void launch_sub_scaled<T>(T*, int64_t, const double*, double, hipStream_t);                         \
+    template void launch_cheby_update<T>(T*, T*, const T*, const T*, const T*, int64_t, double, double, bool,   \
+                                         hipStream_t);                                                           \
     template void launch_interface_sum<T>(const MeshLevel&, int, T*, hipStream_t);                               \
     template void launch_penalty_moment<T>(const MeshLevel&, const T*, double*, hipStream_t);                    \
     template void launch_penalty_init<T>(const MeshLevel&, const double*, const double*, T*, bool, hipStream_t); \

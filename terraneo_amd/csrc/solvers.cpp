@@ -285,21 +285,39 @@ void Chebyshev::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;  // carried across outer iterations, as the reference does
-    for (int it = 0; it < iters_; ++it) {
+    // one step: z = D^-1 (b - A x); d = alpha z + beta d; x += d (fused into one
+    // pass over the vectors unless a custom scaling is set)
+    auto step = [&](double alpha, double beta, bool first) {
         A.apply(c, x, z_);
+        if (!scale) {
+            for (int i = 0; i < x.nb; ++i) {
+                const Field& f = x.b[i];
+                if (f.dtype == TN_F64)
+                    launch_cheby_update<double>((double*)f.p, (double*)d_.b[i].p, (const double*)z_.b[i].p,
+                                                (const double*)b.b[i].p, (const double*)dinv_.b[i].p, f.count(), alpha,
+                                                beta, first, c.s);
+                else
+                    launch_cheby_update<float>((float*)f.p, (float*)d_.b[i].p, (const float*)z_.b[i].p,
+                                               (const float*)b.b[i].p, (const float*)dinv_.b[i].p, f.count(), alpha,
+                                               beta, first, c.s);
+            }
+            return;
+        }
         v_lincomb(c, z_, {1.0, -1.0}, {&b, &z_});
         apply_scale(c, z_);
-        v_lincomb(c, d_, {1.0 / theta}, {&z_});
+        if (first)
+            v_lincomb(c, d_, {alpha}, {&z_});
+        else
+            v_lincomb(c, d_, {alpha, beta}, {&z_, &d_});
         v_lincomb(c, x, {1.0, 1.0}, {&x, &d_});
+    };
+    for (int it = 0; it < iters_; ++it) {
+        step(1.0 / theta, 0.0, true);
         for (int i = 2; i <= order_; ++i) {
             const double rho_new = 1.0 / (2.0 * sigma - rho);
             const double beta = rho_new * rho;
             const double alpha = 2.0 * rho_new / delta;
-            A.apply(c, x, z_);
-            v_lincomb(c, z_, {1.0, -1.0}, {&b, &z_});
-            apply_scale(c, z_);
-            v_lincomb(c, d_, {alpha, beta}, {&z_, &d_});
-            v_lincomb(c, x, {1.0, 1.0}, {&x, &d_});
+            step(alpha, beta, false);
             rho = rho_new;
         }
     }

@@ -134,6 +134,11 @@ void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, dou
 template <typename T>
 void launch_freeslip_project(const MeshLevel& m, T* v, int which, hipStream_t s);
 
+// Chebyshev step: z = dinv (b - az); d = alpha z + beta d (first: alpha z); x += d
+template <typename T>
+void launch_cheby_update(T* x, T* d, const T* az, const T* b, const T* dinv, int64_t count, double alpha, double beta,
+                         bool first, hipStream_t s);
+
 // y[i] -= scale * sum_dev[0], i < count
 template <typename T>
 void launch_sub_scaled(T* y, int64_t count, const double* sum_dev, double scale, hipStream_t s);
