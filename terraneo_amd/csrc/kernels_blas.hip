@@ -94,6 +94,26 @@ __global__ __launch_bounds__(256) void k_cheby_update(T* __restrict__ x, T* __re
     }
 }
 
+// w += alpha x, then out += sum_owned w . y (y == w: the new w's squared norm):
+// one modified Gram-Schmidt step fused with the next step's dot.
+template <typename T, bool SELF>
+__global__ __launch_bounds__(256) void k_axpy_dot(T* __restrict__ w, double alpha, const T* __restrict__ x,
+                                                  const T* __restrict__ y, const uint8_t* __restrict__ owned_lat,
+                                                  int64_t N, int nr, int ncomp, double* out)
+{
+    double s = 0.0;
+    const T a = (T)alpha;
+    const int64_t total = N * ncomp;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const T wn = w[i] + a * x[i];
+        w[i] = wn;
+        const int64_t node = i % N;
+        if (owned_lat[node / nr]) s += (double)wn * (double)(SELF ? wn : y[i]);
+    }
+    block_accumulate(s, out);
+}
+
 // y[i] -= scale * sum[0] (sum: a device scalar from a preceding reduction)
 template <typename T>
 __global__ __launch_bounds__(256) void k_sub_scaled(T* __restrict__ y, int64_t count, const double* __restrict__ sum,
@@ -318,6 +338,20 @@ void launch_cheby_update(T* x, T* d, const T* az, const T* b, const T* dinv, int
 }
 
 template <typename T>
+void launch_axpy_dot(const MeshLevel& m, int ncomp, T* w, double alpha, const T* x, const T* y, double* out_dev,
+                     hipStream_t s)
+{
+    const dim3 gr(grid_for(m.N * ncomp, 256, 2048)), bl(256);
+    if (y == w)
+        hipLaunchKernelGGL((k_axpy_dot<T, true>), gr, bl, 0, s, w, alpha, x, y, m.d_owned_lat.p, m.N, m.nr, ncomp,
+                           out_dev);
+    else
+        hipLaunchKernelGGL((k_axpy_dot<T, false>), gr, bl, 0, s, w, alpha, x, y, m.d_owned_lat.p, m.N, m.nr, ncomp,
+                           out_dev);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
 void launch_sub_scaled(T* y, int64_t count, const double* sum_dev, double scale, hipStream_t s)
 {
     hipLaunchKernelGGL(k_sub_scaled<T>, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, y, count, sum_dev, scale);
@@ -419,6 +453,7 @@ template void launch_cast<float, double>(float*, const double*, int64_t, hipStre
     template void launch_lincomb<T>(int64_t, T*, int, const double*, const T* const*, double, hipStream_t);     \
     template void launch_dot_owned<T>(const MeshLevel&, int, const T*, const T*, double*, hipStream_t);          \
     template void launch_sub_scaled<T>(T*, int64_t, const double*, double, hipStream_t);                         \
+    template void launch_axpy_dot<T>(const MeshLevel&, int, T*, double, const T*, const T*, double*, hipStream_t); \
     template void launch_cheby_update<T>(T*, T*, const T*, const T*, const T*, int64_t, double, double, bool,   \
                                          hipStream_t);                                                           \
     template void launch_interface_sum<T>(const MeshLevel&, int, T*, hipStream_t);                               \

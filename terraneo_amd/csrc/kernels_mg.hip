@@ -407,14 +407,11 @@ __global__ __launch_bounds__(256) void k_stored_cells(StoredArgs<T, S> a)
     const int64_t lat00 = ((int64_t)d * a.n + x) * a.n + y;
     const int64_t latc[4] = {lat00, lat00 + a.n, lat00 + 1, lat00 + a.n + 1};
     T nrm[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    if (a.bc_cmb == TN_FREESLIP || a.bc_surf == TN_FREESLIP) {  // wave-uniform
+    if (a.bc_cmb == TN_FREESLIP || a.bc_surf == TN_FREESLIP) {  // wave-uniform; coordinates are unit vectors
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double* cp = a.coords + 3 * latc[k];
-            const double inv = 1.0 / sqrt(cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2]);
+        for (int k = 0; k < 4; ++k)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) nrm[k][j] = (T)(cp[j] * inv);
-        }
+            for (int j = 0; j < 3; ++j) nrm[k][j] = (T)a.coords[3 * latc[k] + j];
     }
     T yb[4][3], yt[4][3];
 #pragma unroll
@@ -486,9 +483,8 @@ __global__ __launch_bounds__(256) void k_stored_gather(StoredArgs<T, S> a)
                 for (int j = 0; j < 3; ++j) v[j] = bk[j] * x[j];
             }
         } else {
-            const double* cp = a.coords + 3 * lat;
-            const double inv = 1.0 / sqrt(cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2]);
-            double n3[3] = {cp[0] * inv, cp[1] * inv, cp[2] * inv};
+            const double* cp = a.coords + 3 * lat;  // unit vector
+            double n3[3] = {cp[0], cp[1], cp[2]};
             if constexpr (DIAG) {
                 double t1[3], t2[3];
                 nt_basis_d(n3, t1, t2);
@@ -568,9 +564,8 @@ __global__ __launch_bounds__(256) void k_stored_blk(StoredArgs<T, S> a, T* __res
     }
     double e[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
     if (mode == TN_FREESLIP) {
-        const double* cp = a.coords + 3 * lat;
-        const double inv = 1.0 / sqrt(cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2]);
-        for (int i = 0; i < 3; ++i) e[0][i] = cp[i] * inv;
+        const double* cp = a.coords + 3 * lat;  // unit vector
+        for (int i = 0; i < 3; ++i) e[0][i] = cp[i];
         nt_basis_d(e[0], e[1], e[2]);
     }
     for (int b = 0; b < 3; ++b) {
@@ -603,9 +598,8 @@ __global__ __launch_bounds__(256) void k_fs_block_scale(T* __restrict__ z, const
     }
     const int64_t lat = tid / nr;
     const T* bk = blk + ((lev == 0 ? 0 : 10LL * n * n) + lat) * 3;
-    const double* cp = coords + 3 * lat;
-    const double inv = 1.0 / sqrt(cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2]);
-    double n3[3] = {cp[0] * inv, cp[1] * inv, cp[2] * inv}, t1[3], t2[3];
+    const double* cp = coords + 3 * lat;  // unit vector
+    double n3[3] = {cp[0], cp[1], cp[2]}, t1[3], t2[3];
     nt_basis_d(n3, t1, t2);
     const double* rows[3] = {n3, t1, t2};
     const double r[3] = {(double)z[tid], (double)z[N + tid], (double)z[2 * N + tid]};

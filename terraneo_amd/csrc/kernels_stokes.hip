@@ -206,13 +206,6 @@ __device__ inline V3<T> load_coord(const double* coords, int64_t lat)
     return {(T)coords[3 * lat], (T)coords[3 * lat + 1], (T)coords[3 * lat + 2]};
 }
 
-template <typename T>
-__device__ inline V3<T> unit_of(const V3<T>& c)
-{
-    const T inv = T(1) / sqrt(dot3(c, c));
-    return {c.x * inv, c.y * inv, c.z * inv};
-}
-
 // trial-side boundary treatment of a shell node value: Dirichlet drops it,
 // free-slip keeps its tangential part (normal: the node's radial direction)
 template <typename T>
@@ -221,7 +214,7 @@ __device__ inline void trial_bc(V3<T>& u, int mode, const V3<T>& P)
     if (mode == TN_DIRICHLET) {
         u = {T(0), T(0), T(0)};
     } else if (mode == TN_FREESLIP) {
-        const V3<T> nn = unit_of(P);
+        const V3<T>& nn = P;  // lateral coordinates are unit vectors
         const T un = dot3(nn, u);
         u = {u.x - un * nn.x, u.y - un * nn.y, u.z - un * nn.z};
     }
@@ -451,7 +444,7 @@ struct GradCell {
             if (a.add) return;
             v = {T(0), T(0), T(0)};
         } else if (mode == TN_FREESLIP) {
-            const V3<T> nn = unit_of(load_coord<T>(a.coords, lat));
+            const V3<T> nn = load_coord<T>(a.coords, lat);  // unit vector
             const T vn = dot3(nn, v);
             v = {v.x - vn * nn.x, v.y - vn * nn.y, v.z - vn * nn.z};
         }

@@ -13,6 +13,7 @@
 #include "tn_solvers.hpp"
 #include "tn_mg.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <limits>
 
@@ -101,6 +102,21 @@ double v_dot(Ctx& c, const BVec& a, const BVec& b)
             launch_dot_owned<double>(*f.m, f.ncomp, (const double*)f.p, (const double*)b.b[i].p, c.red.p, c.s);
         else
             launch_dot_owned<float>(*f.m, f.ncomp, (const float*)f.p, (const float*)b.b[i].p, c.red.p, c.s);
+    }
+    return c.fetch(1);
+}
+
+double v_axpy_dot(Ctx& c, const BVec& w, double alpha, const BVec& x, const BVec& y)
+{
+    launch_fill<double>(c.red.p, 1, 0.0, c.s);
+    for (int i = 0; i < w.nb; ++i) {
+        const Field& f = w.b[i];
+        if (f.dtype == TN_F64)
+            launch_axpy_dot<double>(*f.m, f.ncomp, (double*)f.p, alpha, (const double*)x.b[i].p,
+                                    (const double*)y.b[i].p, c.red.p, c.s);
+        else
+            launch_axpy_dot<float>(*f.m, f.ncomp, (float*)f.p, alpha, (const float*)x.b[i].p, (const float*)y.b[i].p,
+                                   c.red.p, c.s);
     }
     return c.fetch(1);
 }
@@ -467,13 +483,18 @@ void FGMRES::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
                 ++prec_skipped;
             }
             A.apply(c, Z(j), w);
-            // modified Gram-Schmidt, one dot + axpy per basis vector
+            // modified Gram-Schmidt: h_ij = w . V_i, w -= h_ij V_i for i = 0..j in
+            // turn; each subtraction is fused with the next dot (last: the norm)
+            double hij = v_dot(c, w, V(0));
+            double hn2 = 0.0;
             for (int i = 0; i <= j; ++i) {
-                const double hij = v_dot(c, w, V(i));
                 h(i, j) = hij;
-                v_lincomb(c, w, {1.0, -hij}, {&w, &V(i)});
+                if (i < j)
+                    hij = v_axpy_dot(c, w, -hij, V(i), V(i + 1));
+                else
+                    hn2 = v_axpy_dot(c, w, -hij, V(i), w);
             }
-            const double hn = std::sqrt(v_dot(c, w, w));
+            const double hn = std::sqrt(std::max(hn2, 0.0));
             if (hn < eps * r0) {
                 h(j + 1, j) = 0;
                 inner = j + 1;

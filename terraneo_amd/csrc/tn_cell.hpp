@@ -29,12 +29,14 @@ __device__ inline T dot3(const V3<T>& a, const V3<T>& b)
     return a.x * b.x + a.y * b.y + a.z * b.z;
 }
 
+// Outward unit normal of a shell node: its lateral coordinates, which the mesh
+// generator normalises to the unit sphere (normalising again would change
+// them by at most an ulp, at the cost of a square root and a division in
+// every boundary lane).
 template <typename T>
 __device__ inline V3<T> unit_normal(const T* coords, int64_t lat)
 {
-    const T cx = coords[3 * lat], cy = coords[3 * lat + 1], cz = coords[3 * lat + 2];
-    const T inv = T(1) / sqrt(cx * cx + cy * cy + cz * cz);
-    return {cx * inv, cy * inv, cz * inv};
+    return {coords[3 * lat], coords[3 * lat + 1], coords[3 * lat + 2]};
 }
 
 // Normal-tangential basis (reference local_basis_trafo_normal_tangential.hpp:34-79):

@@ -134,6 +134,10 @@ void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, dou
 template <typename T>
 void launch_freeslip_project(const MeshLevel& m, T* v, int which, hipStream_t s);
 
+// w += alpha x; *out_dev += sum_owned w . y (y == w: squared norm of the new w)
+template <typename T>
+void launch_axpy_dot(const MeshLevel& m, int ncomp, T* w, double alpha, const T* x, const T* y, double* out_dev,
+                     hipStream_t s);
 // Chebyshev step: z = dinv (b - az); d = alpha z + beta d (first: alpha z); x += d
 template <typename T>
 void launch_cheby_update(T* x, T* d, const T* az, const T* b, const T* dinv, int64_t count, double alpha, double beta,

@@ -60,6 +60,8 @@ void v_lincomb(Ctx& c, const BVec& y, std::initializer_list<double> coef, std::i
                double c0 = 0.0);
 void v_assign(Ctx& c, const BVec& y, const BVec& x);
 double v_dot(Ctx& c, const BVec& a, const BVec& b);
+// w += alpha x, then return w . y (owned nodes; y may be w itself)
+double v_axpy_dot(Ctx& c, const BVec& w, double alpha, const BVec& x, const BVec& y);
 void v_mul(Ctx& c, const BVec& y, const BVec& d);
 void v_invert(Ctx& c, const BVec& y);
 void v_randomize(Ctx& c, const BVec& y, uint64_t seed);
