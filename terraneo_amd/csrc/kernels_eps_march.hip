@@ -630,7 +630,9 @@ int march_waves()
     static const int w = [] {
         const char* e = std::getenv("TN_MARCH_WAVES");
         const int v = e ? std::atoi(e) : 0;
-        if (v == 4 || v == 6 || v == 8 || v == 12 || (v == 16 && sizeof(T) == 4)) return v;
+        if (v == 4 || v == 6 || v == 8 || v == 12 || v == 16) return v;
+        // FP64 at 16 waves fits 128 VGPRs only with scratch spills (1.25 ms
+        // against 0.78 ms at 12 waves, level 7 x 129)
         return sizeof(T) == 8 ? 12 : 16;
     }();
     return w;
@@ -646,12 +648,18 @@ void launch_march_w(const MarchArgs<T>& a, bool diagonal, bool tiled, hipStream_
     constexpr int OWN = W - 1;
     const int nxb = (a.n + OWN - 1) / OWN, nys = (a.n + a.seg - 1) / a.seg;
     const dim3 gr((unsigned)(10 * nxb * nys)), bl(64 * W);
-    if (tiled) {
-        if (diagonal)
-            hipLaunchKernelGGL((k_eps_tile<T, true, W>), gr, bl, 0, s, a, a.geo);
-        else
-            hipLaunchKernelGGL((k_eps_tile<T, false, W>), gr, bl, 0, s, a, a.geo);
-    } else if (diagonal)
+    // (the FP64 tile at 16 waves would exceed the LDS: march only there)
+    if constexpr (!(sizeof(T) == 8 && W == 16)) {
+        if (tiled) {
+            if (diagonal)
+                hipLaunchKernelGGL((k_eps_tile<T, true, W>), gr, bl, 0, s, a, a.geo);
+            else
+                hipLaunchKernelGGL((k_eps_tile<T, false, W>), gr, bl, 0, s, a, a.geo);
+            TN_HIP(hipGetLastError());
+            return;
+        }
+    }
+    if (diagonal)
         hipLaunchKernelGGL((k_eps_march<T, true, W>), gr, bl, 0, s, a, a.geo);
     else
         hipLaunchKernelGGL((k_eps_march<T, false, W>), gr, bl, 0, s, a, a.geo);
@@ -664,25 +672,20 @@ void launch_eps_march(const MarchArgs<T>& a, bool diagonal, hipStream_t s)
     TN_REQUIRE(a.nr - 1 >= 2, "march kernel: at least 2 radial cells");
     TN_REQUIRE(a.seg >= 1 && a.seg <= MARCH_SEG_MAX, "march kernel: segment length out of range");
     TN_REQUIRE(10LL * (a.n - 1) * (a.n - 1) * 2 * GEO_STRIDE < (1LL << 31), "march kernel: mesh too large");
-    // FP64: the march kernel (register-staged corner loads); FP32: the LDS-tiled
-    // kernel (measured faster at FP32's higher occupancy).  TN_EPS_KERNEL=march|tile
-    // overrides (tuning).
+    // The march kernel (register-staged corner loads) in both precisions: at
+    // lateral level 7 x 129 it measured 0.78 ms (FP64, 12 waves) and 0.49 ms
+    // (FP32, 16 waves) against 0.98 / 0.50 ms for the LDS-tiled variant.
+    // TN_EPS_KERNEL=march|tile overrides (tuning).
     static const bool tiled = [] {
         const char* e = std::getenv("TN_EPS_KERNEL");
-        if (e && std::string(e) == "tile") return true;
-        if (e && std::string(e) == "march") return false;
-        return sizeof(T) == 4;
+        return e && std::string(e) == "tile";
     }();
     switch (march_waves<T>()) {
         case 4: launch_march_w<T, 4>(a, diagonal, tiled, s); break;
         case 6: launch_march_w<T, 6>(a, diagonal, tiled, s); break;
         case 8: launch_march_w<T, 8>(a, diagonal, tiled, s); break;
         case 12: launch_march_w<T, 12>(a, diagonal, tiled, s); break;
-        default:
-            if constexpr (sizeof(T) == 8)
-                launch_march_w<T, 12>(a, diagonal, tiled, s);  // 16 FP64 tile waves exceed the LDS
-            else
-                launch_march_w<T, 16>(a, diagonal, tiled, s);
+        default: launch_march_w<T, 16>(a, diagonal, tiled, s); break;
             break;
     }
 }
