@@ -88,6 +88,10 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
     constexpr int OWN = W - 1;
     __shared__ T ebuf[2][W][3][65];          // corner (x+1, y) hand-off; slot 64: chunk-top level
     __shared__ T lvl[W][MARCH_SEG_MAX][3];   // chunk-top level values awaiting the next chunk
+    // carried corner sums (x, y+1) and (x+1, y+1) of the previous quad, kept in
+    // LDS (each lane reads back only its own slots) to free registers
+    __shared__ T carry[W][2][3][64];
+    __shared__ T carry_top[W][2][3];        // the same for the chunk-top level (one lane)
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int n = a.n, nr = a.nr, ncr = nr - 1, ncl = n - 1;
     const int nxb = (n + OWN - 1) / OWN, nys = (n + a.seg - 1) / a.seg;
@@ -126,7 +130,12 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
         const T kr = r_mid * r_mid * half_dr * T(1.0 / 6.0);
         const int blev = c == 0 ? bmode : TN_NEUMANN;
         const int tlev = c == ncr - 1 ? tmode : TN_NEUMANN;
-        T c01[3] = {0, 0, 0}, c11[3] = {0, 0, 0}, c01t[3] = {0, 0, 0}, c11t[3] = {0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            carry[w][0][j][lane] = T(0);
+            carry[w][1][j][lane] = T(0);
+            if (lane == 0) carry_top[w][0][j] = carry_top[w][1][j] = T(0);
+        }
 
         for (int qy = y0 - 1; qy <= y1 - 1; ++qy, ++step) {
             const bool quad_ok = strip_ok && qy >= 0 && qy <= n - 2;
@@ -217,18 +226,18 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                     if (lane == 0) up = T(0);
                     M[k] = yb[k][j] + up;
                 }
-                part[j] = M[0] + c01[j];
-                c01[j] = M[2];
-                E[j][lane] = M[1] + c11[j];
-                c11[j] = M[3];
+                part[j] = M[0] + carry[w][0][j][lane];
+                carry[w][0][j][lane] = M[2];
+                E[j][lane] = M[1] + carry[w][1][j][lane];
+                carry[w][1][j][lane] = M[3];
             }
             if (lane == toplane) {  // level above this chunk's last cell
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    tp[j] = yt[0][j] + c01t[j];
-                    c01t[j] = yt[2][j];
-                    E[j][64] = yt[1][j] + c11t[j];
-                    c11t[j] = yt[3][j];
+                    tp[j] = yt[0][j] + carry_top[w][0][j];
+                    carry_top[w][0][j] = yt[2][j];
+                    E[j][64] = yt[1][j] + carry_top[w][1][j];
+                    carry_top[w][1][j] = yt[3][j];
                 }
             }
             __syncthreads();

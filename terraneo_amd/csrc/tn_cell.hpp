@@ -98,6 +98,41 @@ __device__ __forceinline__ void wedge_products(const T U1[3], const T U2[3], con
     q[2] = s02 * zz[0] + s12 * zz[1] + s22 * zz[2];
 }
 
+// The same with the radial scalings applied around the wave-uniform geometry
+// g (l1, l2, z in scalar registers): a1 = alpha l1, a2 = alpha l2,
+// zz = beta z are never formed per lane, which keeps 9 values per wedge out
+// of the vector registers (the scalings move onto U1, U2, D and the outputs).
+template <typename T>
+__device__ __forceinline__ void wedge_products_g(T U1[3], T U2[3], T D[3], T kwJ, const T* __restrict__ g, T alpha,
+                                                 T beta, T p1[3], T p2[3], T q[3])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        U1[k] *= alpha;
+        U2[k] *= alpha;
+        D[k] *= beta;
+    }
+    T G[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) G[k][l] = U1[k] * g[l] + U2[k] * g[3 + l] + D[k] * g[6 + l];
+    const T tr = G[0][0] + G[1][1] + G[2][2];
+    const T k2 = kwJ + kwJ;
+    const T kt = (T(2.0 / 3.0) * kwJ) * tr;
+    const T s00 = k2 * G[0][0] - kt, s11 = k2 * G[1][1] - kt, s22 = k2 * G[2][2] - kt;
+    const T s01 = kwJ * (G[0][1] + G[1][0]), s02 = kwJ * (G[0][2] + G[2][0]), s12 = kwJ * (G[1][2] + G[2][1]);
+    p1[0] = alpha * (s00 * g[0] + s01 * g[1] + s02 * g[2]);
+    p1[1] = alpha * (s01 * g[0] + s11 * g[1] + s12 * g[2]);
+    p1[2] = alpha * (s02 * g[0] + s12 * g[1] + s22 * g[2]);
+    p2[0] = alpha * (s00 * g[3] + s01 * g[4] + s02 * g[5]);
+    p2[1] = alpha * (s01 * g[3] + s11 * g[4] + s12 * g[5]);
+    p2[2] = alpha * (s02 * g[3] + s12 * g[4] + s22 * g[5]);
+    q[0] = beta * (s00 * g[6] + s01 * g[7] + s02 * g[8]);
+    q[1] = beta * (s01 * g[6] + s11 * g[7] + s12 * g[8]);
+    q[2] = beta * (s02 * g[6] + s12 * g[7] + s22 * g[8]);
+}
+
 // Neumann cell (two wedges: corners 0,1,2 and 3,2,1 of the quad 00,10,01,11;
 // kernel_helpers.hpp:302-327).  geo: the two wedges' lateral geometry.
 template <typename T>
@@ -111,9 +146,6 @@ __device__ __forceinline__ void cell_neumann(const CornerIn<T> in[4], const T* _
     T p1a[3], p2a[3], qa[3], p1b[3], p2b[3], qb[3];
     {
         const T* g = geo;
-        const T a1[3] = {alpha * g[0], alpha * g[1], alpha * g[2]};
-        const T a2[3] = {alpha * g[3], alpha * g[4], alpha * g[5]};
-        const T zz[3] = {beta * g[6], beta * g[7], beta * g[8]};
         T U1[3], U2[3], D[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -122,13 +154,10 @@ __device__ __forceinline__ void cell_neumann(const CornerIn<T> in[4], const T* _
             D[k] = in[0].d[k] + d12[k];
         }
         const T kwJ = (in[0].e + e12) * (kr * g[9]);
-        wedge_products(U1, U2, D, kwJ, a1, a2, zz, p1a, p2a, qa);
+        wedge_products_g(U1, U2, D, kwJ, g, alpha, beta, p1a, p2a, qa);
     }
     {
         const T* g = geo + GEO_STRIDE;
-        const T a1[3] = {alpha * g[0], alpha * g[1], alpha * g[2]};
-        const T a2[3] = {alpha * g[3], alpha * g[4], alpha * g[5]};
-        const T zz[3] = {beta * g[6], beta * g[7], beta * g[8]};
         T U1[3], U2[3], D[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -137,7 +166,7 @@ __device__ __forceinline__ void cell_neumann(const CornerIn<T> in[4], const T* _
             D[k] = in[3].d[k] + d12[k];
         }
         const T kwJ = (in[3].e + e12) * (kr * g[9]);
-        wedge_products(U1, U2, D, kwJ, a1, a2, zz, p1b, p2b, qb);
+        wedge_products_g(U1, U2, D, kwJ, g, alpha, beta, p1b, p2b, qb);
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
