@@ -64,7 +64,8 @@ void build_penalty(tn_eps* op)
     S[6] = (G[1][0] * G[2][1] - G[1][1] * G[2][0]) / det;
     S[7] = (G[0][1] * G[2][0] - G[0][0] * G[2][1]) / det;
     S[8] = (G[0][0] * G[1][1] - G[0][1] * G[1][0]) / det;
-    std::vector<double> h(16, 0.0);
+    // [0..3) beta (rotational moments of the input), [4..13) eps S, [16..) the moment pass's partial sums
+    std::vector<double> h(16 + 3 * tn::PENALTY_MAX_BLOCKS, 0.0);
     for (int i = 0; i < 9; ++i) h[4 + i] = op->penalty_eps * S[i];
     op->pen.upload(h);
     op->penalty_ready = true;
@@ -143,8 +144,7 @@ void eps_apply_t(tn_eps* op, const void* src_v, void* dst_v, hipStream_t s)
     const bool add = op->apply_mode == TN_ADD;
     if (penalty) {
         if (!op->penalty_ready) build_penalty(op);
-        tn::launch_fill<double>(op->pen.p, 4, 0.0, s);
-        tn::launch_penalty_moment<T>(m, src, op->pen.p, s);
+        tn::launch_penalty_moment<T>(m, src, op->pen.p, op->pen.p + 16, s);
     }
     if (op->stored_dtype < 0) {
         // matrix-free, atomic-free: the kernel writes every node copy once

@@ -63,17 +63,52 @@ __device__ inline void block_accumulate(double v, double* out)
     }
 }
 
+// Owned-node reductions walk an [ncomp][nlat][nr] field by radial columns:
+// one wave per column (c, lat), lanes over the column's nr contiguous entries
+// (coalesced), the lateral ownership flag read once per column.  No per-entry
+// index division.
+struct ColumnWalk {
+    int64_t col, ncol, step;
+    int64_t nlat;
+    __device__ ColumnWalk(int64_t nlat_, int ncomp)
+        : col((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)),
+          ncol(nlat_ * ncomp),
+          step((int64_t)gridDim.x * (blockDim.x >> 6)),
+          nlat(nlat_)
+    {
+    }
+};
+
+// Block total of `v`, valid in thread 0 (no atomics).
+__device__ inline double block_total(double v)
+{
+    __shared__ double part[16];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) part[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (wv == 0) {
+        const int nw = blockDim.x >> 6;
+        s = lane < nw ? part[lane] : 0.0;
+        s = wave_sum(s);
+    }
+    __syncthreads();
+    return s;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_dot_owned(const T* __restrict__ a, const T* __restrict__ b,
                                                    const uint8_t* __restrict__ owned_lat, int64_t N, int nr,
                                                    int ncomp, double* out)
 {
     double s = 0.0;
-    const int64_t total = N * ncomp;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t node = i % N;
-        if (owned_lat[node / nr]) s += (double)a[i] * (double)b[i];
+    const int lane = threadIdx.x & 63;
+    for (ColumnWalk cw(N / nr, ncomp); cw.col < cw.ncol; cw.col += cw.step) {
+        if (!owned_lat[cw.col % cw.nlat]) continue;
+        const T* ac = a + cw.col * nr;
+        const T* bc = b + cw.col * nr;
+        for (int r = lane; r < nr; r += 64) s += (double)ac[r] * (double)bc[r];
     }
     block_accumulate(s, out);
 }
@@ -103,13 +138,17 @@ __global__ __launch_bounds__(256) void k_axpy_dot(T* __restrict__ w, double alph
 {
     double s = 0.0;
     const T a = (T)alpha;
-    const int64_t total = N * ncomp;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const T wn = w[i] + a * x[i];
-        w[i] = wn;
-        const int64_t node = i % N;
-        if (owned_lat[node / nr]) s += (double)wn * (double)(SELF ? wn : y[i]);
+    const int lane = threadIdx.x & 63;
+    for (ColumnWalk cw(N / nr, ncomp); cw.col < cw.ncol; cw.col += cw.step) {
+        const bool own = owned_lat[cw.col % cw.nlat];
+        T* wc = w + cw.col * nr;
+        const T* xc = x + cw.col * nr;
+        const T* yc = y + cw.col * nr;
+        for (int r = lane; r < nr; r += 64) {
+            const T wn = wc[r] + a * xc[r];
+            wc[r] = wn;
+            if (own) s += (double)wn * (double)(SELF ? wn : yc[r]);
+        }
     }
     block_accumulate(s, out);
 }
@@ -141,29 +180,91 @@ __global__ void k_interface_sum(const int32_t* __restrict__ off, const int32_t* 
     }
 }
 
-// beta += sum over owned nodes of (x_node) cross u_node
+// beta += sum over owned nodes of (x_node) cross u_node.  A wave walks radial
+// columns (lateral nodes) two at a time, lanes over the radial nodes, up to
+// three per lane per pass.  coords_owned holds the column's unit position at owned
+// lateral nodes and 0 at the other copies, so every load of a pass (the
+// coordinates, radii, the three components) is unconditional and independent
+// of the others: a column costs one memory round trip.
 template <typename T>
-__global__ __launch_bounds__(256) void k_penalty_moment(const T* __restrict__ u, const double* __restrict__ coords,
-                                                        const double* __restrict__ radii,
-                                                        const uint8_t* __restrict__ owned_lat, int64_t N, int nr,
-                                                        double* beta)
+__global__ __launch_bounds__(256) void k_penalty_moment(const T* __restrict__ u,
+                                                        const double* __restrict__ coords_owned,
+                                                        const double* __restrict__ radii, int64_t N, int nr,
+                                                        double* __restrict__ partial)
 {
+    constexpr int COLS = 2;  // columns in flight per wave
     double m0 = 0, m1 = 0, m2 = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t lat = i / nr;
-        if (!owned_lat[lat]) continue;
-        const double r = radii[i % nr];
-        const double px = coords[3 * lat] * r, py = coords[3 * lat + 1] * r, pz = coords[3 * lat + 2] * r;
-        const double ux = u[i], uy = u[N + i], uz = u[2 * N + i];
-        m0 += py * uz - pz * uy;
-        m1 += pz * ux - px * uz;
-        m2 += px * uy - py * ux;
+    const int lane = threadIdx.x & 63;
+    const int64_t nlat = N / nr;
+    // wave-uniform column index (scalar registers: scalar loads of the coordinates)
+    const int64_t wave0 =
+        (int64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t lat0 = wave0; lat0 < nlat; lat0 += COLS * waves) {
+        for (int r0 = 0; r0 < nr; r0 += 192) {
+            // clamped indices keep every load unconditional; columns past the mesh
+            // and slots past the column get weight 0
+            double c[COLS][3], rr[3], wt[COLS][3];
+            T vx[COLS][3], vy[COLS][3], vz[COLS][3];
+#pragma unroll
+            for (int q = 0; q < COLS; ++q) {
+                const int64_t lat = lat0 + q * waves;
+                const int64_t latc = lat < nlat ? lat : nlat - 1;
+                const double lw = lat < nlat ? 1.0 : 0.0;
+                c[q][0] = coords_owned[3 * latc];
+                c[q][1] = coords_owned[3 * latc + 1];
+                c[q][2] = coords_owned[3 * latc + 2];
+                const T* ux = u + latc * nr;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int r = r0 + 64 * k + lane;
+                    const int rc = r < nr ? r : nr - 1;
+                    wt[q][k] = r < nr ? lw : 0.0;
+                    if (q == 0) rr[k] = radii[rc];
+                    vx[q][k] = ux[rc];
+                    vy[q][k] = ux[N + rc];
+                    vz[q][k] = ux[2 * N + rc];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);  // every load of the pass is issued before the arithmetic
+#pragma unroll
+            for (int q = 0; q < COLS; ++q)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const double w = wt[q][k] * rr[k];
+                    const double px = c[q][0] * w, py = c[q][1] * w, pz = c[q][2] * w;
+                    const double ax = vx[q][k], ay = vy[q][k], az = vz[q][k];
+                    m0 += py * az - pz * ay;
+                    m1 += pz * ax - px * az;
+                    m2 += px * ay - py * ax;
+                }
+        }
     }
-    block_accumulate(m0, beta);
-    __syncthreads();
-    block_accumulate(m1, beta + 1);
-    __syncthreads();
-    block_accumulate(m2, beta + 2);
+    // per-block partial sums, added up by k_penalty_finalize: one store per
+    // block instead of three same-address atomics, which serialise in L2
+    m0 = block_total(m0);
+    m1 = block_total(m1);
+    m2 = block_total(m2);
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x] = m0;
+        partial[3 * blockIdx.x + 1] = m1;
+        partial[3 * blockIdx.x + 2] = m2;
+    }
+}
+
+// beta[k] = sum over the nblk partial sums (one block)
+__global__ __launch_bounds__(256) void k_penalty_finalize(const double* __restrict__ partial, int nblk,
+                                                          double* __restrict__ beta)
+{
+    double m[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) m[k] += partial[3 * b + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double t = block_total(m[k]);
+        if (threadIdx.x == 0) beta[k] = t;
+    }
 }
 
 // y (+)= gamma x p, gamma = S beta (S: 3x3 row-major)
@@ -261,14 +362,16 @@ __global__ __launch_bounds__(256) void k_multidot(const T* __restrict__ a, Multi
     double s[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) s[k] = 0.0;
-    const int64_t total = N * ncomp;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        if (!owned_lat[(i % N) / nr]) continue;
-        const double av = (double)a[i];
+    const int lane = threadIdx.x & 63;
+    for (ColumnWalk cw(N / nr, ncomp); cw.col < cw.ncol; cw.col += cw.step) {
+        if (!owned_lat[cw.col % cw.nlat]) continue;
+        const int64_t base = cw.col * nr;
+        for (int r = lane; r < nr; r += 64) {
+            const double av = (double)a[base + r];
 #pragma unroll
-        for (int k = 0; k < NB; ++k)
-            if (k < bs.nb) s[k] += av * (double)bs.b[k][i];
+            for (int k = 0; k < NB; ++k)
+                if (k < bs.nb) s[k] += av * (double)bs.b[k][base + r];
+        }
     }
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
@@ -320,7 +423,7 @@ void launch_lincomb(int64_t count, T* y, int nx, const double* c, const T* const
 template <typename T>
 void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, double* out_dev, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dot_owned<T>, dim3(grid_for(m.N * ncomp, 256, 2048)), dim3(256), 0, s, a, b,
+    hipLaunchKernelGGL(k_dot_owned<T>, dim3(grid_for(m.N / m.nr * ncomp, 4, 2048)), dim3(256), 0, s, a, b,
                        m.d_owned_lat.p, m.N, m.nr, ncomp, out_dev);
     TN_HIP(hipGetLastError());
 }
@@ -341,7 +444,7 @@ template <typename T>
 void launch_axpy_dot(const MeshLevel& m, int ncomp, T* w, double alpha, const T* x, const T* y, double* out_dev,
                      hipStream_t s)
 {
-    const dim3 gr(grid_for(m.N * ncomp, 256, 2048)), bl(256);
+    const dim3 gr(grid_for(m.N / m.nr * ncomp, 4, 2048)), bl(256);
     if (y == w)
         hipLaunchKernelGGL((k_axpy_dot<T, true>), gr, bl, 0, s, w, alpha, x, y, m.d_owned_lat.p, m.N, m.nr, ncomp,
                            out_dev);
@@ -370,10 +473,13 @@ void launch_interface_sum(const MeshLevel& m, int ncomp, T* v, hipStream_t s)
 }
 
 template <typename T>
-void launch_penalty_moment(const MeshLevel& m, const T* u, double* beta_dev, hipStream_t s)
+void launch_penalty_moment(const MeshLevel& m, const T* u, double* beta_dev, double* partial_dev, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_penalty_moment<T>, dim3(grid_for(m.N, 256, 2048)), dim3(256), 0, s, u, m.d_coords.p,
-                       m.d_radii.p, m.d_owned_lat.p, m.N, m.nr, beta_dev);
+    const unsigned g = grid_for(m.N / m.nr, 4, PENALTY_MAX_BLOCKS);
+    hipLaunchKernelGGL(k_penalty_moment<T>, dim3(g), dim3(256), 0, s, u, m.d_coords_owned.p, m.d_radii.p, m.N, m.nr,
+                       partial_dev);
+    TN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_penalty_finalize, dim3(1), dim3(256), 0, s, partial_dev, (int)g, beta_dev);
     TN_HIP(hipGetLastError());
 }
 
@@ -434,7 +540,7 @@ void launch_multidot(const MeshLevel& m, int ncomp, const T* a, const MultiDotAr
                      hipStream_t s)
 {
     TN_REQUIRE(bs.nb >= 1 && bs.nb <= MultiDotArgs<T>::MAX, "multidot: 1..16 vectors");
-    const dim3 gr(grid_for(m.N * ncomp, 256, 1024)), bl(256);
+    const dim3 gr(grid_for(m.N / m.nr * ncomp, 4, 1024)), bl(256);
     if (bs.nb <= 4)
         hipLaunchKernelGGL((k_multidot<T, 4>), gr, bl, 0, s, a, bs, m.d_owned_lat.p, m.N, m.nr, ncomp, out_dev);
     else if (bs.nb <= 8)
@@ -457,7 +563,7 @@ template void launch_cast<float, double>(float*, const double*, int64_t, hipStre
     template void launch_cheby_update<T>(T*, T*, const T*, const T*, const T*, int64_t, double, double, bool,   \
                                          hipStream_t);                                                           \
     template void launch_interface_sum<T>(const MeshLevel&, int, T*, hipStream_t);                               \
-    template void launch_penalty_moment<T>(const MeshLevel&, const T*, double*, hipStream_t);                    \
+    template void launch_penalty_moment<T>(const MeshLevel&, const T*, double*, double*, hipStream_t);                    \
     template void launch_penalty_init<T>(const MeshLevel&, const double*, const double*, T*, bool, hipStream_t); \
     template void launch_mul<T>(T*, const T*, int64_t, hipStream_t);                                             \
     template void launch_max_abs<T>(const T*, int64_t, double*, hipStream_t);                                    \

@@ -46,15 +46,14 @@ namespace {
 using namespace cellops;
 
 // Final value of a boundary node copy from its Neumann-formula value y (of the
-// zeroed / projected source).  blk: the copy's three quadratic forms of the
+// zeroed / projected source) and its source value x.  blk: the copy's three quadratic forms of the
 // 3x3 diagonal block B of the Neumann operator, e^T B e for e = x, y, z axes
 // (Dirichlet) or e = n, t1, t2 (free-slip).  Linear in y and blk, so summing
 // the copies afterwards gives the full node's value.
 template <typename T, bool DIAG>
-__device__ inline V3<T> test_value(V3<T> y, const T* __restrict__ src, int64_t N, int64_t idx, int mode,
-                                   const T* coords, int64_t lat, const T* __restrict__ blk)
+__device__ inline V3<T> test_value(V3<T> y, V3<T> x, int mode, const T* coords, int64_t lat,
+                                   const T* __restrict__ blk)
 {
-    const V3<T> x{src[idx], src[N + idx], src[2 * N + idx]};
     if (mode == TN_DIRICHLET) {
         // DIAG: the Neumann diagonal of the raw source is already diag(B) x
         if constexpr (DIAG) return y;
@@ -83,8 +82,13 @@ __device__ __forceinline__ T shfl_up1(T v)
 }
 
 template <typename T, bool DIAG, int W>
-__global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* __restrict__ geo_tab)
+__global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* __restrict__ geo_tab, const T* __restrict__ coords,
+                                                          const T* __restrict__ blk_tab,
+                                                          const double* __restrict__ coords_d,
+                                                          const double* __restrict__ pen)
 {
+    // the read-only tables indexed by the (wave-uniform) lateral node are
+    // separate restrict parameters, so their loads can go through the scalar unit
     constexpr int OWN = W - 1;
     __shared__ T ebuf[2][W][3][65];          // corner (x+1, y) hand-off; slot 64: chunk-top level
     __shared__ T lvl[W][MARCH_SEG_MAX][3];   // chunk-top level values awaiting the next chunk
@@ -109,9 +113,9 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
     const int nchunks = (ncr + 63) >> 6;
     // fs/fs rotation penalty: y += gamma x p, gamma = S beta (double, as the separate penalty pass)
     double gam[3] = {0, 0, 0};
-    if (a.pen) {
-        const double* bt = a.pen;
-        const double* S = a.pen + 4;
+    if (pen) {
+        const double* bt = pen;
+        const double* S = pen + 4;
         gam[0] = S[0] * bt[0] + S[1] * bt[1] + S[2] * bt[2];
         gam[1] = S[3] * bt[0] + S[4] * bt[1] + S[5] * bt[2];
         gam[2] = S[6] * bt[0] + S[7] * bt[1] + S[8] * bt[2];
@@ -130,6 +134,9 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
         const T kr = r_mid * r_mid * half_dr * T(1.0 / 6.0);
         const int blev = c == 0 ? bmode : TN_NEUMANN;
         const int tlev = c == ncr - 1 ? tmode : TN_NEUMANN;
+        // rotation penalty: radii of this lane's node level and of the top level
+        const double prad = pen ? a.radii_d[cc] : 0.0;
+        const double prad_top = pen ? a.radii_d[ncr] : 0.0;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             carry[w][0][j][lane] = T(0);
@@ -146,6 +153,10 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
             for (int k = 0; k < 4; ++k)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
+            // the source value at node (x, qy) of a boundary lane (level 0 or the top
+            // level), kept from the cell's own loads for the test side of the store
+            T xn[3] = {0, 0, 0};
+            const bool have_xn = quad_ok && ncr > 1;
             if (quad_ok) {
                 // wave-uniform: scalar loads into SGPRs
                 const int goff = __builtin_amdgcn_readfirstlane(((d * ncl + x) * ncl + qy) * 2 * GEO_STRIDE);
@@ -175,6 +186,8 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                                 in[k].et = a.eta[ib + 1];
                             }
                         }
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) xn[j] = c == 0 ? in[0].b[j] : in[0].t[j];
                         cell_neumann_diag<T>(in, geo, alpha, beta, kr, yb, yt);
                     } else {
                         // issue every load of the cell before any lane-divergent boundary work
@@ -194,11 +207,13 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                                 et[k] = a.eta[ib + 1];
                             }
                         }
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) xn[j] = c == 0 ? xb[0][j] : xt[0][j];
                         if (blev != TN_NEUMANN || tlev != TN_NEUMANN) {
 #pragma unroll
                             for (int k = 0; k < 4; ++k) {
-                                if (blev != TN_NEUMANN) apply_trial(xb[k], blev, a.coords, latc[k]);
-                                if (tlev != TN_NEUMANN) apply_trial(xt[k], tlev, a.coords, latc[k]);
+                                if (blev != TN_NEUMANN) apply_trial(xb[k], blev, coords, latc[k]);
+                                if (tlev != TN_NEUMANN) apply_trial(xt[k], tlev, coords, latc[k]);
                             }
                         }
                         CornerIn<T> in[4];
@@ -244,16 +259,19 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
             if (row_own && qy >= y0) {
                 const T(*Ein)[65] = ebuf[step & 1][w - 1];
                 const int64_t lat = lat00;
-                const T* blk0 = a.blk ? a.blk + lat * 3 : nullptr;                   // CMB shell
-                const T* blk1 = a.blk ? a.blk + (10LL * n * n + lat) * 3 : nullptr;  // surface shell
+                const T* blk0 = blk_tab ? blk_tab + lat * 3 : nullptr;                   // CMB shell
+                const T* blk1 = blk_tab ? blk_tab + (10LL * n * n + lat) * 3 : nullptr;  // surface shell
                 const int slot = qy - y0;
-                auto finish = [&](V3<T> v, int level, int mode, const T* blk) {
+                auto finish = [&](V3<T> v, int level, int mode, const T* blk, double r) {
                     const int64_t idx = lat * nr + level;
-                    if (mode != TN_NEUMANN) v = test_value<T, DIAG>(v, a.src, N, idx, mode, a.coords, lat, blk);
-                    if (a.pen) {
-                        const double r = a.radii_d[level];
-                        const double px = a.coords_d[3 * lat] * r, py = a.coords_d[3 * lat + 1] * r,
-                                     pz = a.coords_d[3 * lat + 2] * r;
+                    if (mode != TN_NEUMANN) {
+                        const V3<T> xv = have_xn ? V3<T>{xn[0], xn[1], xn[2]}
+                                                 : V3<T>{a.src[idx], a.src[N + idx], a.src[2 * N + idx]};
+                        v = test_value<T, DIAG>(v, xv, mode, coords, lat, blk);
+                    }
+                    if (pen) {
+                        const double px = coords_d[3 * lat] * r, py = coords_d[3 * lat + 1] * r,
+                                     pz = coords_d[3 * lat + 2] * r;
                         v.x += (T)(gam[1] * pz - gam[2] * py);
                         v.y += (T)(gam[2] * px - gam[0] * pz);
                         v.z += (T)(gam[0] * py - gam[1] * px);
@@ -275,12 +293,12 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                         v.y += lvl[w][slot][1];
                         v.z += lvl[w][slot][2];
                     }
-                    finish(v, c, c == 0 ? bmode : TN_NEUMANN, blk0);
+                    finish(v, c, c == 0 ? bmode : TN_NEUMANN, blk0, prad);
                 }
                 if (lane == toplane) {
                     const V3<T> v{tp[0] + Ein[0][64], tp[1] + Ein[1][64], tp[2] + Ein[2][64]};
                     if (last) {
-                        finish(v, ncr, tmode, blk1);
+                        finish(v, ncr, tmode, blk1, prad_top);
                     } else {
                         lvl[w][slot][0] = v.x;
                         lvl[w][slot][1] = v.y;
@@ -490,7 +508,9 @@ __global__ __launch_bounds__(64 * W) void k_eps_tile(MarchArgs<T> a, const T* __
                 const int slot = qy - y0;
                 auto finish = [&](V3<T> v, int level, int mode, const T* blk) {
                     const int64_t idx = lat * nr + level;
-                    if (mode != TN_NEUMANN) v = test_value<T, DIAG>(v, a.src, N, idx, mode, a.coords, lat, blk);
+                    if (mode != TN_NEUMANN)
+                        v = test_value<T, DIAG>(v, V3<T>{a.src[idx], a.src[N + idx], a.src[2 * N + idx]}, mode,
+                                                a.coords, lat, blk);
                     if (a.pen) {
                         const double r = a.radii_d[level];
                         const double px = a.coords_d[3 * lat] * r, py = a.coords_d[3 * lat + 1] * r,
@@ -669,9 +689,9 @@ void launch_march_w(const MarchArgs<T>& a, bool diagonal, bool tiled, hipStream_
         }
     }
     if (diagonal)
-        hipLaunchKernelGGL((k_eps_march<T, true, W>), gr, bl, 0, s, a, a.geo);
+        hipLaunchKernelGGL((k_eps_march<T, true, W>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen);
     else
-        hipLaunchKernelGGL((k_eps_march<T, false, W>), gr, bl, 0, s, a, a.geo);
+        hipLaunchKernelGGL((k_eps_march<T, false, W>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen);
     TN_HIP(hipGetLastError());
 }
 

@@ -221,6 +221,10 @@ void MeshLevel::build(int lat_level_in, const std::vector<double>& radii_in)
 void MeshLevel::upload()
 {
     d_coords.upload(coords);
+    std::vector<double> co(coords);
+    for (size_t lat = 0; lat < owned_lat.size(); ++lat)
+        if (!owned_lat[lat]) co[3 * lat] = co[3 * lat + 1] = co[3 * lat + 2] = 0.0;
+    d_coords_owned.upload(co);
     std::vector<float> cf(coords.begin(), coords.end());
     d_coords_f.upload(cf);
     d_radii.upload(radii);

@@ -82,6 +82,7 @@ struct MeshLevel {
     std::vector<double> geo;        // [10][n-1][n-1][2][GEO_STRIDE]
 
     DevBuf<double> d_coords;
+    DevBuf<double> d_coords_owned;  // coords of owned lateral nodes, 0 at the other copies
     DevBuf<float> d_coords_f;
     DevBuf<double> d_radii;
     DevBuf<float> d_radii_f;
@@ -149,7 +150,9 @@ void launch_sub_scaled(T* y, int64_t count, const double* sum_dev, double scale,
 
 // fs/fs penalty: beta[0..3) = sum_owned r x u  ;  y (+)= gamma x r with gamma = S beta
 template <typename T>
-void launch_penalty_moment(const MeshLevel& m, const T* u, double* beta_dev, hipStream_t s);
+void launch_penalty_moment(const MeshLevel& m, const T* u, double* beta_dev, double* partial_dev, hipStream_t s);
+// blocks of the moment pass = partial sums it leaves in partial_dev (3 doubles each)
+constexpr int PENALTY_MAX_BLOCKS = 2048;
 template <typename T>
 void launch_penalty_init(const MeshLevel& m, const double* beta_dev, const double* S_dev, T* y, bool add,
                          hipStream_t s);

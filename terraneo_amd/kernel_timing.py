@@ -7,6 +7,11 @@
     python -m terraneo_amd.kernel_timing coupling [--lat 7] [--reps 20]
         B, B^T (Replace and Add) and the full Stokes block apply, velocity
         lateral level lat x (2^lat + 1) radial nodes, free-slip shells
+    python -m terraneo_amd.kernel_timing eps [--lat 7] [--reps 20]
+        the matrix-free epsilon-div-div apply (Replace, no interface sum) at
+        lateral level lat x (2^lat + 1) radial nodes for each boundary-condition
+        pair, FP64 and FP32; free-slip on both shells includes the rotation
+        penalty (its moment pass and the fused store)
 
 Each line reports the mean wall time per call over `reps` calls after three
 warm-up calls (stream-synchronised), so that a `rocprofv3 --kernel-trace
@@ -18,7 +23,7 @@ import time
 import torch
 
 from . import api
-from ._capi import TN_ADD, TN_FREESLIP
+from ._capi import TN_ADD, TN_COMM_SKIP, TN_DIRICHLET, TN_FREESLIP, TN_NEUMANN, TN_REPLACE
 from .problem import StokesProblem
 
 
@@ -71,13 +76,30 @@ def coupling(lat, reps):
         print(f"velocity level {lat}: {name}: {_time(fn, reps):.3f} ms per call", flush=True)
 
 
+def eps(lat, reps):
+    dev = torch.device("cuda:0")
+    m = api.Mesh(lat, lat, 0.55, 1.0)
+    names = {TN_NEUMANN: "neumann", TN_DIRICHLET: "dirichlet", TN_FREESLIP: "free-slip"}
+    for dt in (torch.float64, torch.float32):
+        eta = torch.ones(m.N, dtype=dt, device=dev)
+        x = torch.randn(3 * m.N, dtype=dt, device=dev)
+        y = torch.empty_like(x)
+        for bc in ((TN_NEUMANN, TN_NEUMANN), (TN_DIRICHLET, TN_DIRICHLET), (TN_FREESLIP, TN_DIRICHLET),
+                   (TN_FREESLIP, TN_FREESLIP)):
+            A = api.Eps(m, eta, *bc)
+            A.set_modes(TN_REPLACE, TN_COMM_SKIP)
+            ms = _time(lambda: A.apply(x, y), reps)
+            print(f"level {lat} eps {str(dt)[6:]} {names[bc[0]]}/{names[bc[1]]}: {ms:.3f} ms per apply", flush=True)
+            del A
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["stored", "coupling"])
+    ap.add_argument("what", choices=["stored", "coupling", "eps"])
     ap.add_argument("--lat", type=int, default=7)
     ap.add_argument("--reps", type=int, default=10)
     a = ap.parse_args()
-    (stored if a.what == "stored" else coupling)(a.lat, a.reps)
+    {"stored": stored, "coupling": coupling, "eps": eps}[a.what](a.lat, a.reps)
 
 
 if __name__ == "__main__":
