@@ -75,6 +75,22 @@ __device__ inline V3<T> test_value(V3<T> y, V3<T> x, int mode, const T* coords, 
     }
 }
 
+// Point-to-point hand-off between the waves of a block through LDS counters
+// (no block-wide barrier per step, so the waves of a SIMD drift apart and hide
+// each other's memory latency).  The wait is bounded: it always ends.
+__device__ __forceinline__ void wait_counter(int* c, int target)
+{
+    for (int i = 0; i < (1 << 16); ++i) {
+        if (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return;
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__device__ __forceinline__ void post_counter(int* c, int value, int lane)
+{
+    if (lane == 0) __hip_atomic_store(c, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 template <typename T>
 __device__ __forceinline__ T shfl_up1(T v)
 {
@@ -90,7 +106,11 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
     // the read-only tables indexed by the (wave-uniform) lateral node are
     // separate restrict parameters, so their loads can go through the scalar unit
     constexpr int OWN = W - 1;
-    __shared__ T ebuf[2][W][3][65];          // corner (x+1, y) hand-off; slot 64: chunk-top level
+    // hand-off slots per wave: a producer may lead its consumer by RING - 1 steps
+    // (FP64 at 16 waves: 3, to stay within 160 KB of LDS)
+    constexpr int RING = (sizeof(T) == 8 && W > 12) ? 3 : 4;
+    __shared__ T ebuf[RING][W][3][65];       // corner (x+1, y) hand-off; slot 64: chunk-top level
+    __shared__ int done[W], used[W];         // steps whose hand-off wave w has written / has read
     __shared__ T lvl[W][MARCH_SEG_MAX][3];   // chunk-top level values awaiting the next chunk
     // carried corner sums (x, y+1) and (x+1, y+1) of the previous quad, kept in
     // LDS (each lane reads back only its own slots) to free registers
@@ -120,7 +140,9 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
         gam[1] = S[3] * bt[0] + S[4] * bt[1] + S[5] * bt[2];
         gam[2] = S[6] * bt[0] + S[7] * bt[1] + S[8] * bt[2];
     }
-    int step = 0;  // LDS double-buffer parity, continuous over chunks
+    if (lane == 0) done[w] = used[w] = 0;
+    __syncthreads();
+    int step = 0;  // hand-off ring position, continuous over chunks
 
     for (int h = 0; h < nchunks; ++h) {
         const int c = 64 * h + lane;
@@ -146,7 +168,7 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
 
         for (int qy = y0 - 1; qy <= y1 - 1; ++qy, ++step) {
             const bool quad_ok = strip_ok && qy >= 0 && qy <= n - 2;
-            T(*E)[65] = ebuf[step & 1][w];
+            T(*E)[65] = ebuf[step % RING][w];
             const int64_t lat00 = ((int64_t)d * n + x) * n + qy;
             T yb[4][3], yt[4][3];
 #pragma unroll
@@ -230,7 +252,9 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                     }
                 }
             }
-            // radial merge (node level c = bottom of cell c + top of cell c-1) and routing
+            // radial merge (node level c = bottom of cell c + top of cell c-1) and routing;
+            // the slot is free once the next wave has read what it held RING steps ago
+            if (w + 1 < W) wait_counter(&used[w + 1], step - RING + 1);
             T part[3], tp[3] = {0, 0, 0};
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
@@ -255,9 +279,10 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                     carry_top[w][1][j] = yt[3][j];
                 }
             }
-            __syncthreads();
+            post_counter(&done[w], step + 1, lane);
+            if (w >= 1) wait_counter(&done[w - 1], step + 1);
             if (row_own && qy >= y0) {
-                const T(*Ein)[65] = ebuf[step & 1][w - 1];
+                const T(*Ein)[65] = ebuf[step % RING][w - 1];
                 const int64_t lat = lat00;
                 const T* blk0 = blk_tab ? blk_tab + lat * 3 : nullptr;                   // CMB shell
                 const T* blk1 = blk_tab ? blk_tab + (10LL * n * n + lat) * 3 : nullptr;  // surface shell
@@ -306,6 +331,7 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                     }
                 }
             }
+            post_counter(&used[w], step + 1, lane);
         }
     }
 }
@@ -649,10 +675,12 @@ void launch_boundary_blocks(const Geom<T>& g, const T* eta, const T* eta_cell, i
     TN_HIP(hipGetLastError());
 }
 
-// Waves per block (W-1 node rows per block).  FP64: 168 VGPRs -> 3 waves/SIMD
-// -> W = 12 (one block per CU); FP32: <= 128 VGPRs -> 4 waves/SIMD -> W = 16.  More waves per block means a smaller
-// share of redundant halo strips (W/(W-1)); more resident blocks per CU hide
-// the per-step barrier.  TN_MARCH_WAVES overrides (tuning: 4, 6, 8, 12, 16).
+// Waves per block (W-1 node rows per block), one block per CU (LDS).  With
+// the counter hand-off the kernel needs 115 (FP64) / 72 (FP32) VGPRs; measured
+// at level 7 x 129 (Dirichlet apply): FP64 0.665 ms at W = 16 against 0.725 ms
+// at 12, FP32 0.422 ms at W = 12 against 0.489 ms at 16.  More waves per block
+// mean a smaller share of redundant halo strips (W/(W-1)).  TN_MARCH_WAVES
+// overrides (tuning: 4, 6, 8, 12, 16).
 template <typename T>
 int march_waves()
 {
@@ -660,16 +688,14 @@ int march_waves()
         const char* e = std::getenv("TN_MARCH_WAVES");
         const int v = e ? std::atoi(e) : 0;
         if (v == 4 || v == 6 || v == 8 || v == 12 || v == 16) return v;
-        // FP64 at 16 waves fits 128 VGPRs only with scratch spills (1.25 ms
-        // against 0.78 ms at 12 waves, level 7 x 129)
-        return sizeof(T) == 8 ? 12 : 16;
+        return sizeof(T) == 8 ? 16 : 12;
     }();
     return w;
 }
 
 int march_block_rows(int dtype_size) { return (dtype_size == 8 ? march_waves<double>() : march_waves<float>()) - 1; }
 
-int march_waves_per_cu(int dtype_size) { return dtype_size == 8 ? 12 : 16; }
+int march_waves_per_cu(int dtype_size) { return march_block_rows(dtype_size) + 1; }
 
 template <typename T, int W>
 void launch_march_w(const MarchArgs<T>& a, bool diagonal, bool tiled, hipStream_t s)
