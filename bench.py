@@ -91,6 +91,7 @@ def solve_bench(args):
         log(f"solve {prec}: {info['iterations']} iterations, {info['seconds']:.3f} s")
         res = pr.residual(u, p, bu, bp, pressure_mean_free=True)
         out[prec] = {"seconds": round(info["seconds"], 4), "iterations": info["iterations"],
+                     "ms_per_iteration": round(info["seconds"] / max(info["iterations"], 1) * 1e3, 3),
                      "true_rel_residual": float(f"{res:.3e}"), "setup_seconds": round(setup, 3)}
         if prec == "f64":
             sols["u"] = u.clone()
@@ -113,6 +114,7 @@ def solve_bench(args):
         "fp64_gmg": out["f64"],
         "mixed_gmg": out["f32"],
         "mixed_over_fp64_time": round(out["f32"]["seconds"] / out["f64"]["seconds"], 4),
+        "mixed_over_fp64_time_per_iteration": round(out["f32"]["ms_per_iteration"] / out["f64"]["ms_per_iteration"], 4),
         "velocity_rel_l2_vs_fp64": out["velocity_rel_l2_vs_fp64"],
     }
 

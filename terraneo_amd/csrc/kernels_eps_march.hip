@@ -39,6 +39,7 @@
 
 #include "tn_cell.hpp"
 #include "tn_eps_march.hpp"
+#include "tn_march.hpp"
 
 namespace tn {
 namespace {
@@ -73,22 +74,6 @@ __device__ inline V3<T> test_value(V3<T> y, V3<T> x, int mode, const T* coords, 
         const T corr = blk[0] * dot3(nn, x) - dot3(nn, y);
         return {y.x + corr * nn.x, y.y + corr * nn.y, y.z + corr * nn.z};
     }
-}
-
-// Point-to-point hand-off between the waves of a block through LDS counters
-// (no block-wide barrier per step, so the waves of a SIMD drift apart and hide
-// each other's memory latency).  The wait is bounded: it always ends.
-__device__ __forceinline__ void wait_counter(int* c, int target)
-{
-    for (int i = 0; i < (1 << 16); ++i) {
-        if (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return;
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
-
-__device__ __forceinline__ void post_counter(int* c, int value, int lane)
-{
-    if (lane == 0) __hip_atomic_store(c, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 template <typename T>
@@ -254,7 +239,7 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
             }
             // radial merge (node level c = bottom of cell c + top of cell c-1) and routing;
             // the slot is free once the next wave has read what it held RING steps ago
-            if (w + 1 < W) wait_counter(&used[w + 1], step - RING + 1);
+            if (w + 1 < W) march::wait_counter(&used[w + 1], step - RING + 1);
             T part[3], tp[3] = {0, 0, 0};
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
@@ -279,8 +264,8 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                     carry_top[w][1][j] = yt[3][j];
                 }
             }
-            post_counter(&done[w], step + 1, lane);
-            if (w >= 1) wait_counter(&done[w - 1], step + 1);
+            march::post_counter(&done[w], step + 1, lane);
+            if (w >= 1) march::wait_counter(&done[w - 1], step + 1);
             if (row_own && qy >= y0) {
                 const T(*Ein)[65] = ebuf[step % RING][w - 1];
                 const int64_t lat = lat00;
@@ -331,7 +316,7 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                     }
                 }
             }
-            post_counter(&used[w], step + 1, lane);
+            march::post_counter(&used[w], step + 1, lane);
         }
     }
 }

@@ -387,10 +387,18 @@ __global__ __launch_bounds__(256) void k_div(DivArgs<T> a)
 template <typename T>
 struct GradCell {
     const GradArgs<T> a;  // by value: a reference would put the kernel argument in scratch
+    // the arrays, taken from restrict kernel parameters: loads at wave-uniform
+    // addresses (the lateral coordinates) can then go through the scalar unit
+    const double* coords;
+    const T* p;
+    T* u;
     int64_t N;
     RadMoments<T> rm;
     int R;
-    __device__ GradCell(const GradArgs<T>& args) : a(args), N(10LL * args.nf * args.nf * args.nrf) {}
+    __device__ GradCell(const GradArgs<T>& args, const double* c, const T* pp, T* uu)
+        : a(args), coords(c), p(pp), u(uu), N(10LL * args.nf * args.nf * args.nrf)
+    {
+    }
     __device__ void begin_chunk(int, int cc, bool valid)
     {
         rm = radial_moments<T>(a.radii[cc], a.radii[cc + 1], cc & 1, valid ? 1.0 : 0.0);
@@ -404,14 +412,14 @@ struct GradCell {
         T pc[4][2];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            pc[k][0] = a.p[cl[k] * nrc + R];
-            pc[k][1] = a.p[cl[k] * nrc + R + 1];
+            pc[k][0] = p[cl[k] * nrc + R];
+            pc[k][1] = p[cl[k] * nrc + R + 1];
         }
         const int64_t lat00 = ((int64_t)d * n + x) * n + qy;
         const int64_t latc[4] = {lat00, lat00 + n, lat00 + 1, lat00 + n + 1};
         V3<T> P[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) P[k] = load_coord<T>(a.coords, latc[k]);
+        for (int k = 0; k < 4; ++k) P[k] = load_coord<T>(coords, latc[k]);
 #pragma unroll
         for (int w2 = 0; w2 < 2; ++w2) {
             const int k0 = WK[w2][0], k1 = WK[w2][1], k2 = WK[w2][2];
@@ -444,26 +452,27 @@ struct GradCell {
             if (a.add) return;
             v = {T(0), T(0), T(0)};
         } else if (mode == TN_FREESLIP) {
-            const V3<T> nn = load_coord<T>(a.coords, lat);  // unit vector
+            const V3<T> nn = load_coord<T>(coords, lat);  // unit vector
             const T vn = dot3(nn, v);
             v = {v.x - vn * nn.x, v.y - vn * nn.y, v.z - vn * nn.z};
         }
         if (a.add) {
-            a.u[idx] += v.x;
-            a.u[N + idx] += v.y;
-            a.u[2 * N + idx] += v.z;
+            u[idx] += v.x;
+            u[N + idx] += v.y;
+            u[2 * N + idx] += v.z;
         } else {
-            a.u[idx] = v.x;
-            a.u[N + idx] = v.y;
-            a.u[2 * N + idx] = v.z;
+            u[idx] = v.x;
+            u[N + idx] = v.y;
+            u[2 * N + idx] = v.z;
         }
     }
 };
 
 template <typename T, int W>
-__global__ __launch_bounds__(64 * W) void k_grad_march(GradArgs<T> a)
+__global__ __launch_bounds__(64 * W) void k_grad_march(GradArgs<T> a, const double* __restrict__ coords,
+                                                         const T* __restrict__ p, T* __restrict__ u)
 {
-    GradCell<T> op(a);
+    GradCell<T> op(a, coords, p, u);
     march::run<T, W>(op, a.nf, a.nrf, a.seg);
 }
 
@@ -550,7 +559,7 @@ void launch_grad(const GradArgs<T>& a, hipStream_t s)
     TN_REQUIRE(a.seg >= 1 && a.seg <= MARCH_SEG_MAX, "B^T: segment length out of range");
     constexpr int W = GRAD_MARCH_WAVES;
     hipLaunchKernelGGL((k_grad_march<T, W>), dim3((unsigned)march::blocks_for(a.nf, a.seg, W)), dim3(64 * W), 0, s,
-                       a);
+                       a, a.coords, a.p, a.u);
     TN_HIP(hipGetLastError());
 }
 

@@ -9,7 +9,8 @@
 // the other).  A quad's sums at its four lateral corners are routed
 //   corner (x,   y)   own node row, column y   -> complete after this step
 //   corner (x,   y+1) own node row, column y+1 -> carried in registers
-//   corner (x+1, y)   next wave's row          -> LDS, read by wave w+1
+//   corner (x+1, y)   next wave's row          -> LDS ring, read by wave w+1
+//                                                 (LDS counters order producer and consumer)
 //   corner (x+1, y+1)                          -> carried, sent next step
 // and radially adjacent cells share their node level through a lane shuffle;
 // the level above a chunk's last cell goes the same way through that lane and
@@ -42,12 +43,30 @@ __host__ __device__ inline int blocks_for(int n, int seg, int waves)
     return 10 * ((n + own - 1) / own) * ((n + seg - 1) / seg);
 }
 
+// Point-to-point hand-off between neighbouring waves through LDS counters
+// instead of a block barrier per step: the waves of a SIMD drift apart and hide
+// each other's memory latency.  The wait is bounded, so it always ends.
+__device__ __forceinline__ void wait_counter(int* c, int target)
+{
+    for (int i = 0; i < (1 << 16); ++i) {
+        if (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return;
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__device__ __forceinline__ void post_counter(int* c, int value, int lane)
+{
+    if (lane == 0) __hip_atomic_store(c, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 template <typename T, int W, class Op>
 __device__ __forceinline__ void run(Op& op, int n, int nr, int seg)
 {
     constexpr int OWN = W - 1;
-    __shared__ T ebuf[2][W][3][65];         // corner (x+1, y) hand-off; slot 64: chunk-top level
-    __shared__ T lvl[W][MARCH_SEG_MAX][3];  // chunk-top level values awaiting the next chunk
+    constexpr int RING = 4;                  // hand-off slots: a producer may lead by RING - 1 steps
+    __shared__ T ebuf[RING][W][3][65];       // corner (x+1, y) hand-off; slot 64: chunk-top level
+    __shared__ T lvl[W][MARCH_SEG_MAX][3];   // chunk-top level values awaiting the next chunk
+    __shared__ int done[W], used[W];         // steps whose hand-off wave w has written / has read
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int ncr = nr - 1;
     const int nxb = (n + OWN - 1) / OWN, nys = (n + seg - 1) / seg;
@@ -61,7 +80,9 @@ __device__ __forceinline__ void run(Op& op, int n, int nr, int seg)
     const bool row_own = w >= 1 && x <= n - 1;
     const int y0 = ys * seg, y1 = min(y0 + seg, n);
     const int nchunks = (ncr + 63) >> 6;
-    int step = 0;  // LDS double-buffer parity, continuous over chunks
+    if (lane == 0) done[w] = used[w] = 0;
+    __syncthreads();
+    int step = 0;  // hand-off ring position, continuous over chunks
 
     for (int h = 0; h < nchunks; ++h) {
         const int c = 64 * h + lane;
@@ -73,14 +94,16 @@ __device__ __forceinline__ void run(Op& op, int n, int nr, int seg)
 
         for (int qy = y0 - 1; qy <= y1 - 1; ++qy, ++step) {
             const bool quad_ok = strip_ok && qy >= 0 && qy <= n - 2;
-            T(*E)[65] = ebuf[step & 1][w];
+            T(*E)[65] = ebuf[step % RING][w];
             T yb[4][3], yt[4][3];
 #pragma unroll
             for (int k = 0; k < 4; ++k)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
             if (quad_ok) op.cell(d, x, qy, yb, yt);
-            // radial merge (node level c = bottom of cell c + top of cell c-1) and routing
+            // radial merge (node level c = bottom of cell c + top of cell c-1) and routing;
+            // the slot is free once the next wave has read what it held RING steps ago
+            if (w + 1 < W) wait_counter(&used[w + 1], step - RING + 1);
             T part[3], tp[3] = {0, 0, 0};
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
@@ -105,9 +128,10 @@ __device__ __forceinline__ void run(Op& op, int n, int nr, int seg)
                     c11t[j] = yt[3][j];
                 }
             }
-            __syncthreads();
+            post_counter(&done[w], step + 1, lane);
+            if (w >= 1) wait_counter(&done[w - 1], step + 1);
             if (row_own && qy >= y0) {
-                const T(*Ein)[65] = ebuf[step & 1][w - 1];
+                const T(*Ein)[65] = ebuf[step % RING][w - 1];
                 const int64_t lat = ((int64_t)d * n + x) * n + qy;
                 const int slot = qy - y0;
                 if (valid) {
@@ -130,6 +154,7 @@ __device__ __forceinline__ void run(Op& op, int n, int nr, int seg)
                     }
                 }
             }
+            post_counter(&used[w], step + 1, lane);
         }
     }
 }

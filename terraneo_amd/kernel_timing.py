@@ -7,6 +7,13 @@
     python -m terraneo_amd.kernel_timing coupling [--lat 7] [--reps 20]
         B, B^T (Replace and Add) and the full Stokes block apply, velocity
         lateral level lat x (2^lat + 1) radial nodes, free-slip shells
+    python -m terraneo_amd.kernel_timing solve [--lat 7] [--reps 10]
+        the free-slip Stokes solve of the mixed-precision workload (viscosity
+        contrast 1e5 x 30, 5-level GMG, Chebyshev degree 3 x 2, FGMRES(30),
+        mean-free pressure): after the setup, `reps` FGMRES iterations in FP64
+        and then with the FP32 V-cycle.  A small torch sort launch brackets
+        each timed solve, so a kernel trace can be cut into per-iteration
+        kernel costs
     python -m terraneo_amd.kernel_timing eps [--lat 7] [--reps 20]
         the matrix-free epsilon-div-div apply (Replace, no interface sum) at
         lateral level lat x (2^lat + 1) radial nodes for each boundary-condition
@@ -93,13 +100,46 @@ def eps(lat, reps):
             del A
 
 
+def solve(lat, reps):
+    import gc
+
+    from . import workloads as W
+
+    dev = torch.device("cuda:0")
+
+    def marker():
+        torch.cuda.synchronize()
+        torch.sort(torch.rand(100, device=dev))
+        torch.cuda.synchronize()
+
+    for prec in ("f64", "f32"):
+        pr = StokesProblem(lat, lat, 5, device=dev, prec_dtype=prec,
+                           gmg=api.gmg_options(cheby_order=3, cheby_iterations=2, power_iterations=10,
+                                               coarse_solver="dense"))
+        coords, radii = pr.coords()
+        pr.set_viscosity(eta=torch.from_numpy(W.viscosity_c2(coords, radii).reshape(-1)).to(dev))
+        bu, bp = pr.rhs_from_force(torch.from_numpy(W.buoyancy(coords, radii)).to(dev))
+        pr.solve(bu, bp, rtol=1e-8, max_iterations=1, pressure_mean_free=True)  # setup: eigenvalue estimates
+        marker()
+        t0 = time.perf_counter()
+        u, p, info = pr.solve(bu, bp, rtol=1e-8, max_iterations=reps, restart=30, pressure_mean_free=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        marker()
+        print(f"velocity level {lat} solve {prec}: {info['iterations']} FGMRES iterations, "
+              f"{dt / info['iterations'] * 1e3:.2f} ms per iteration", flush=True)
+        del pr, u, p, bu, bp
+        gc.collect()
+        torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["stored", "coupling", "eps"])
+    ap.add_argument("what", choices=["stored", "coupling", "eps", "solve"])
     ap.add_argument("--lat", type=int, default=7)
     ap.add_argument("--reps", type=int, default=10)
     a = ap.parse_args()
-    {"stored": stored, "coupling": coupling, "eps": eps}[a.what](a.lat, a.reps)
+    {"stored": stored, "coupling": coupling, "eps": eps, "solve": solve}[a.what](a.lat, a.reps)
 
 
 if __name__ == "__main__":
