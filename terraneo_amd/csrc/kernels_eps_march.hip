@@ -7,10 +7,13 @@
 // the other), so every global load and store is radially contiguous.  A
 // quad's contributions to its four lateral corners are routed as follows:
 //   corner (x,   y)   own node row, column y   -> complete after this step
-//   corner (x,   y+1) own node row, column y+1 -> carried in registers
-//   corner (x+1, y)   next wave's row          -> LDS, read by wave w+1
+//   corner (x,   y+1) own node row, column y+1 -> carried (own LDS slots)
+//   corner (x+1, y)   next wave's row          -> LDS ring, read by wave w+1
 //   corner (x+1, y+1)                          -> carried, sent next step
 // and radially adjacent cells share their node level through a lane shuffle.
+// Waves hand off through LDS counters (written / read steps per wave), not a
+// block barrier, so the waves of a SIMD drift apart by up to the ring depth
+// and hide each other's load latency.
 // The node level above a chunk's last cell is routed the same way by that
 // lane; it is either the surface (stored) or the first level of the next
 // chunk (kept in LDS until that chunk reaches the column).  Every node copy
@@ -712,10 +715,11 @@ void launch_eps_march(const MarchArgs<T>& a, bool diagonal, hipStream_t s)
     TN_REQUIRE(a.nr - 1 >= 2, "march kernel: at least 2 radial cells");
     TN_REQUIRE(a.seg >= 1 && a.seg <= MARCH_SEG_MAX, "march kernel: segment length out of range");
     TN_REQUIRE(10LL * (a.n - 1) * (a.n - 1) * 2 * GEO_STRIDE < (1LL << 31), "march kernel: mesh too large");
-    // The march kernel (register-staged corner loads) in both precisions: at
-    // lateral level 7 x 129 it measured 0.78 ms (FP64, 12 waves) and 0.49 ms
-    // (FP32, 16 waves) against 0.98 / 0.50 ms for the LDS-tiled variant.
-    // TN_EPS_KERNEL=march|tile overrides (tuning).
+    // The march kernel in both precisions: at lateral level 7 x 129 it
+    // measured 0.64 ms (FP64, 16 waves) and 0.42 ms (FP32, 12 waves); the
+    // LDS-tiled variant (barrier per step) measured 0.98 / 0.50 ms against the
+    // barrier version of the march kernel.  TN_EPS_KERNEL=march|tile overrides
+    // (tuning).
     static const bool tiled = [] {
         const char* e = std::getenv("TN_EPS_KERNEL");
         return e && std::string(e) == "tile";
