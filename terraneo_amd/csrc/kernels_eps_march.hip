@@ -38,7 +38,6 @@
 // 2 x 89 (wedges: 9 scalings, 9 sums, 27 gradient, 14 stress, 27 products,
 // 3 kwJ) + 39 (outputs) + 15 (merge, routing) = 264.
 #include <cstdlib>
-#include <string>
 
 #include "tn_cell.hpp"
 #include "tn_eps_march.hpp"
@@ -324,249 +323,6 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
     }
 }
 
-template <typename T, bool DIAG, int W>
-__global__ __launch_bounds__(64 * W) void k_eps_tile(MarchArgs<T> a, const T* __restrict__ geo_tab)
-{
-    constexpr int OWN = W - 1;
-    // node-column tiles: rows x0-1 .. x0+W-1, a ring of 3 columns, fields x0 x1 x2 eta, 65 levels
-    __shared__ T tile[W + 1][3][4][66];
-    __shared__ T ebuf[2][W][3][65];          // corner (x+1, y) hand-off; slot 64: chunk-top level
-    __shared__ T lvl[W][MARCH_SEG_MAX][3];   // chunk-top level values awaiting the next chunk
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int n = a.n, nr = a.nr, ncr = nr - 1, ncl = n - 1;
-    const int nxb = (n + OWN - 1) / OWN, nys = (n + a.seg - 1) / a.seg;
-    int b = blockIdx.x;
-    const int ys = b % nys;
-    b /= nys;
-    const int xb = b % nxb;
-    const int d = b / nxb;
-    const int x = xb * OWN - 1 + w;
-    const bool strip_ok = x >= 0 && x <= n - 2;
-    const bool row_own = w >= 1 && x <= n - 1;
-    const int y0 = ys * a.seg, y1 = min(y0 + a.seg, n);
-    const int64_t N = 10LL * n * n * nr;
-    const int bmode = a.bc_cmb, tmode = a.bc_surf;
-    const int nchunks = (ncr + 63) >> 6;
-    // fs/fs rotation penalty: y += gamma x p, gamma = S beta (double, as the separate penalty pass)
-    double gam[3] = {0, 0, 0};
-    if (a.pen) {
-        const double* bt = a.pen;
-        const double* S = a.pen + 4;
-        gam[0] = S[0] * bt[0] + S[1] * bt[1] + S[2] * bt[2];
-        gam[1] = S[3] * bt[0] + S[4] * bt[1] + S[5] * bt[2];
-        gam[2] = S[6] * bt[0] + S[7] * bt[1] + S[8] * bt[2];
-    }
-    int step = 0;  // LDS double-buffer parity, continuous over chunks
-    const int xrow = xb * OWN - 1 + w;  // node row this wave stages (tile row w)
-    const int xextra = xb * OWN - 1 + W;  // tile row W, staged field-wise by waves 0..3
-
-    for (int h = 0; h < nchunks; ++h) {
-        const int c = 64 * h + lane;
-        const bool valid = c < ncr;
-        const int toplane = min(63, ncr - 1 - 64 * h);
-        const bool last = h == nchunks - 1;
-        const int cc = valid ? c : ncr - 1;
-        const T r0 = a.radii[cc], r1 = a.radii[cc + 1];
-        const T half_dr = T(0.5) * (r1 - r0), r_mid = T(0.5) * (r0 + r1);
-        const T alpha = T(1) / r_mid, beta = T(1) / half_dr;
-        // lanes past the last radial cell compute on clamped data with zero weight
-        const T kr = valid ? r_mid * r_mid * half_dr * T(1.0 / 6.0) : T(0);
-        const int blev = c == 0 ? bmode : TN_NEUMANN;
-        const int tlev = c == ncr - 1 ? tmode : TN_NEUMANN;
-        T c01[3] = {0, 0, 0}, c11[3] = {0, 0, 0}, c01t[3] = {0, 0, 0}, c11t[3] = {0, 0, 0};
-        const int lev = 64 * h + lane;           // staged node level of this lane
-        const bool lev_ok = lev <= ncr;
-        const bool top_ok = lane == 0 && 64 * h + 64 <= ncr;
-        // stage node column yc of this wave's row and its share of the extra row into registers
-        T cv[4], cvt[4], ev[2];
-        auto fetch = [&](int yc) {
-            const bool rok = xrow >= 0 && xrow <= n - 1 && yc >= 0 && yc <= n - 1;
-            const int64_t base = (((int64_t)d * n + xrow) * n + yc) * nr;
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                const T* src = f < 3 ? a.src + f * N : a.eta;
-                cv[f] = (rok && lev_ok && (f < 3 || a.eta)) ? src[base + lev] : T(0);
-                cvt[f] = (rok && top_ok && (f < 3 || a.eta)) ? src[base + 64 * h + 64] : T(0);
-            }
-            const bool eok = w < 4 && xextra <= n - 1 && yc >= 0 && yc <= n - 1 && (w < 3 || a.eta);
-            const int64_t ebase = (((int64_t)d * n + xextra) * n + yc) * nr;
-            const T* esrc = w < 3 ? a.src + w * N : a.eta;
-            ev[0] = (eok && lev_ok) ? esrc[ebase + lev] : T(0);
-            ev[1] = (eok && top_ok) ? esrc[ebase + 64 * h + 64] : T(0);
-        };
-        auto commit = [&](int yc) {
-            const int sl = (yc + 3) % 3;
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                tile[w][sl][f][lane] = cv[f];
-                if (lane == 0) tile[w][sl][f][64] = cvt[f];
-            }
-            if (w < 4) {
-                tile[W][sl][w][lane] = ev[0];
-                if (lane == 0) tile[W][sl][w][64] = ev[1];
-            }
-        };
-        fetch(y0 - 1);
-        commit(y0 - 1);
-        fetch(y0);
-        commit(y0);
-        fetch(y0 + 1);  // committed at the first step
-        __syncthreads();
-
-        for (int qy = y0 - 1; qy <= y1 - 1; ++qy, ++step) {
-            const bool quad_ok = strip_ok && qy >= 0 && qy <= n - 2;
-            T(*E)[65] = ebuf[step & 1][w];
-            commit(qy + 2);  // fetched one step ago; the ring slot of column qy - 1, read by nobody now
-            fetch(qy + 3);   // in flight for a whole step
-            const int64_t lat00 = ((int64_t)d * n + x) * n + qy;
-            T yb[4][3], yt[4][3];
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
-            if (quad_ok) {
-                // wave-uniform: scalar loads into SGPRs
-                const int goff = __builtin_amdgcn_readfirstlane(((d * ncl + x) * ncl + qy) * 2 * GEO_STRIDE);
-                const T* gsrc = geo_tab + goff;
-                T geo[2 * GEO_STRIDE];
-#pragma unroll
-                for (int i = 0; i < 2 * GEO_STRIDE; ++i) geo[i] = gsrc[i];
-                if (valid) {
-                    const int64_t latc[4] = {lat00, lat00 + n, lat00 + 1, lat00 + n + 1};
-                    const T ecell = a.eta_cell ? a.eta_cell[(((int64_t)d * ncl + x) * ncl + qy) * ncr + c] : T(0);
-                    const int s0 = (qy + 3) % 3, s1 = (qy + 4) % 3;
-                    const int trow[4] = {w, w + 1, w, w + 1}, tsl[4] = {s0, s0, s1, s1};
-                    if constexpr (DIAG) {
-                        CornerRaw<T> in[4];
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const T(*tk)[66] = tile[trow[k]][tsl[k]];
-#pragma unroll
-                            for (int j = 0; j < 3; ++j) {
-                                in[k].b[j] = tk[j][lane];
-                                in[k].t[j] = tk[j][lane + 1];
-                            }
-                            if (a.eta_cell) {
-                                in[k].eb = in[k].et = ecell;
-                            } else {
-                                in[k].eb = tk[3][lane];
-                                in[k].et = tk[3][lane + 1];
-                            }
-                        }
-                        cell_neumann_diag<T>(in, geo, alpha, beta, kr, yb, yt);
-                    } else {
-                        T xb[4][3], xt[4][3], eb[4], et[4];
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const T(*tk)[66] = tile[trow[k]][tsl[k]];
-#pragma unroll
-                            for (int j = 0; j < 3; ++j) {
-                                xb[k][j] = tk[j][lane];
-                                xt[k][j] = tk[j][lane + 1];
-                            }
-                            eb[k] = a.eta_cell ? ecell : tk[3][lane];
-                            et[k] = a.eta_cell ? ecell : tk[3][lane + 1];
-                        }
-                        if (blev != TN_NEUMANN || tlev != TN_NEUMANN) {
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) {
-                                if (blev != TN_NEUMANN) apply_trial(xb[k], blev, a.coords, latc[k]);
-                                if (tlev != TN_NEUMANN) apply_trial(xt[k], tlev, a.coords, latc[k]);
-                            }
-                        }
-                        CornerIn<T> in[4];
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-#pragma unroll
-                            for (int j = 0; j < 3; ++j) {
-                                in[k].S[j] = xb[k][j] + xt[k][j];
-                                in[k].d[j] = xt[k][j] - xb[k][j];
-                            }
-                            in[k].e = eb[k] + et[k];
-                        }
-                        cell_neumann<T>(in, geo, alpha, beta, kr, yb, yt);
-                    }
-                }
-            }
-            // radial merge (node level c = bottom of cell c + top of cell c-1) and routing
-            T part[3], tp[3] = {0, 0, 0};
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                T M[4];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    T up = shfl_up1(yt[k][j]);
-                    if (lane == 0) up = T(0);
-                    M[k] = yb[k][j] + up;
-                }
-                part[j] = M[0] + c01[j];
-                c01[j] = M[2];
-                E[j][lane] = M[1] + c11[j];
-                c11[j] = M[3];
-            }
-            if (lane == toplane) {  // level above this chunk's last cell
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    tp[j] = yt[0][j] + c01t[j];
-                    c01t[j] = yt[2][j];
-                    E[j][64] = yt[1][j] + c11t[j];
-                    c11t[j] = yt[3][j];
-                }
-            }
-            __syncthreads();
-            if (row_own && qy >= y0) {
-                const T(*Ein)[65] = ebuf[step & 1][w - 1];
-                const int64_t lat = lat00;
-                const T* blk0 = a.blk ? a.blk + lat * 3 : nullptr;                   // CMB shell
-                const T* blk1 = a.blk ? a.blk + (10LL * n * n + lat) * 3 : nullptr;  // surface shell
-                const int slot = qy - y0;
-                auto finish = [&](V3<T> v, int level, int mode, const T* blk) {
-                    const int64_t idx = lat * nr + level;
-                    if (mode != TN_NEUMANN)
-                        v = test_value<T, DIAG>(v, V3<T>{a.src[idx], a.src[N + idx], a.src[2 * N + idx]}, mode,
-                                                a.coords, lat, blk);
-                    if (a.pen) {
-                        const double r = a.radii_d[level];
-                        const double px = a.coords_d[3 * lat] * r, py = a.coords_d[3 * lat + 1] * r,
-                                     pz = a.coords_d[3 * lat + 2] * r;
-                        v.x += (T)(gam[1] * pz - gam[2] * py);
-                        v.y += (T)(gam[2] * px - gam[0] * pz);
-                        v.z += (T)(gam[0] * py - gam[1] * px);
-                    }
-                    if (a.add) {
-                        a.dst[idx] += v.x;
-                        a.dst[N + idx] += v.y;
-                        a.dst[2 * N + idx] += v.z;
-                    } else {
-                        a.dst[idx] = v.x;
-                        a.dst[N + idx] = v.y;
-                        a.dst[2 * N + idx] = v.z;
-                    }
-                };
-                if (valid) {
-                    V3<T> v{part[0] + Ein[0][lane], part[1] + Ein[1][lane], part[2] + Ein[2][lane]};
-                    if (lane == 0 && h > 0) {
-                        v.x += lvl[w][slot][0];
-                        v.y += lvl[w][slot][1];
-                        v.z += lvl[w][slot][2];
-                    }
-                    finish(v, c, c == 0 ? bmode : TN_NEUMANN, blk0);
-                }
-                if (lane == toplane) {
-                    const V3<T> v{tp[0] + Ein[0][64], tp[1] + Ein[1][64], tp[2] + Ein[2][64]};
-                    if (last) {
-                        finish(v, ncr, tmode, blk1);
-                    } else {
-                        lvl[w][slot][0] = v.x;
-                        lvl[w][slot][1] = v.y;
-                        lvl[w][slot][2] = v.z;
-                    }
-                }
-            }
-        }
-    }
-}
-
 // Quadratic forms of the 3x3 diagonal block of the Neumann operator at every
 // boundary node copy, from the copy's own diamond's cells (gather over the up
 // to four adjacent quads, no atomics):
@@ -686,22 +442,11 @@ int march_block_rows(int dtype_size) { return (dtype_size == 8 ? march_waves<dou
 int march_waves_per_cu(int dtype_size) { return march_block_rows(dtype_size) + 1; }
 
 template <typename T, int W>
-void launch_march_w(const MarchArgs<T>& a, bool diagonal, bool tiled, hipStream_t s)
+void launch_march_w(const MarchArgs<T>& a, bool diagonal, hipStream_t s)
 {
     constexpr int OWN = W - 1;
     const int nxb = (a.n + OWN - 1) / OWN, nys = (a.n + a.seg - 1) / a.seg;
     const dim3 gr((unsigned)(10 * nxb * nys)), bl(64 * W);
-    // (the FP64 tile at 16 waves would exceed the LDS: march only there)
-    if constexpr (!(sizeof(T) == 8 && W == 16)) {
-        if (tiled) {
-            if (diagonal)
-                hipLaunchKernelGGL((k_eps_tile<T, true, W>), gr, bl, 0, s, a, a.geo);
-            else
-                hipLaunchKernelGGL((k_eps_tile<T, false, W>), gr, bl, 0, s, a, a.geo);
-            TN_HIP(hipGetLastError());
-            return;
-        }
-    }
     if (diagonal)
         hipLaunchKernelGGL((k_eps_march<T, true, W>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen);
     else
@@ -715,21 +460,16 @@ void launch_eps_march(const MarchArgs<T>& a, bool diagonal, hipStream_t s)
     TN_REQUIRE(a.nr - 1 >= 2, "march kernel: at least 2 radial cells");
     TN_REQUIRE(a.seg >= 1 && a.seg <= MARCH_SEG_MAX, "march kernel: segment length out of range");
     TN_REQUIRE(10LL * (a.n - 1) * (a.n - 1) * 2 * GEO_STRIDE < (1LL << 31), "march kernel: mesh too large");
-    // The march kernel in both precisions: at lateral level 7 x 129 it
-    // measured 0.64 ms (FP64, 16 waves) and 0.42 ms (FP32, 12 waves); the
-    // LDS-tiled variant (barrier per step) measured 0.98 / 0.50 ms against the
-    // barrier version of the march kernel.  TN_EPS_KERNEL=march|tile overrides
-    // (tuning).
-    static const bool tiled = [] {
-        const char* e = std::getenv("TN_EPS_KERNEL");
-        return e && std::string(e) == "tile";
-    }();
+    // at lateral level 7 x 129 the kernel measured 0.64 ms (FP64, 16 waves)
+    // and 0.42 ms (FP32, 12 waves); an LDS-tiled variant that staged the
+    // source columns in LDS with a barrier per step was slower (0.98 / 0.50 ms
+    // against 0.78 / 0.49 ms for the barrier form of this kernel) and is gone
     switch (march_waves<T>()) {
-        case 4: launch_march_w<T, 4>(a, diagonal, tiled, s); break;
-        case 6: launch_march_w<T, 6>(a, diagonal, tiled, s); break;
-        case 8: launch_march_w<T, 8>(a, diagonal, tiled, s); break;
-        case 12: launch_march_w<T, 12>(a, diagonal, tiled, s); break;
-        default: launch_march_w<T, 16>(a, diagonal, tiled, s); break;
+        case 4: launch_march_w<T, 4>(a, diagonal, s); break;
+        case 6: launch_march_w<T, 6>(a, diagonal, s); break;
+        case 8: launch_march_w<T, 8>(a, diagonal, s); break;
+        case 12: launch_march_w<T, 12>(a, diagonal, s); break;
+        default: launch_march_w<T, 16>(a, diagonal, s); break;
             break;
     }
 }

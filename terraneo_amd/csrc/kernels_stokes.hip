@@ -469,7 +469,8 @@ struct GradCell {
 };
 
 template <typename T, int W>
-__global__ __launch_bounds__(64 * W) void k_grad_march(GradArgs<T> a, const double* __restrict__ coords,
+// (3 waves per SIMD: the FP64 build fits 168 VGPRs without spills)
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3))) void k_grad_march(GradArgs<T> a, const double* __restrict__ coords,
                                                          const T* __restrict__ p, T* __restrict__ u)
 {
     GradCell<T> op(a, coords, p, u);
@@ -557,7 +558,7 @@ void launch_grad(const GradArgs<T>& a, hipStream_t s)
     TN_REQUIRE(a.nf == 2 * a.nc - 1 && a.nrf == 2 * a.nrc - 1 && a.nrc >= 2 && a.nc >= 2,
                "B^T: velocity mesh must be the pressure mesh refined once");
     TN_REQUIRE(a.seg >= 1 && a.seg <= MARCH_SEG_MAX, "B^T: segment length out of range");
-    constexpr int W = GRAD_MARCH_WAVES;
+    constexpr int W = grad_march_waves<T>();
     hipLaunchKernelGGL((k_grad_march<T, W>), dim3((unsigned)march::blocks_for(a.nf, a.seg, W)), dim3(64 * W), 0, s,
                        a, a.coords, a.p, a.u);
     TN_HIP(hipGetLastError());

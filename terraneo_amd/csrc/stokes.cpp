@@ -42,8 +42,10 @@ void grad_t(tn_stokes* K, const void* p, void* u, bool add, bool comm, hipStream
 {
     auto& mf = const_cast<tn_mesh*>(K->vel)->lvl.dev();
     auto& mc = const_cast<tn_mesh*>(K->pre)->lvl.dev();
-    GradArgs<T> a{mf.d_coords.p, mf.d_radii.p, mf.n, mf.nr, mc.n, mc.nr, K->A->bc_cmb, K->A->bc_surf, (const T*)p, (T*)u,
-                  march_seg_for(mf.n, GRAD_MARCH_WAVES - 1, sizeof(T) == 8 ? GRAD_MARCH_WAVES : 2 * GRAD_MARCH_WAVES), add ? 1 : 0};
+    constexpr int W = grad_march_waves<T>();
+    const int seg = march_seg_for(mf.n, W - 1, sizeof(T) == 8 ? W : 2 * W);
+    GradArgs<T> a{mf.d_coords.p, mf.d_radii.p, mf.n,          mf.nr,        mc.n, mc.nr, K->A->bc_cmb,
+                  K->A->bc_surf, (const T*)p,   (T*)u, seg, add ? 1 : 0};
     launch_grad<T>(a, s);
     if (comm) launch_interface_sum<T>(mf, 3, (T*)u, s);
 }

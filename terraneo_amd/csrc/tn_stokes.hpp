@@ -27,8 +27,13 @@ struct GradArgs {
     int add;     // 1: u += B^T p, 0: u = B^T p
 };
 
-// waves per block of the B^T march (node rows per block + 1)
-constexpr int GRAD_MARCH_WAVES = 8;
+// waves per block of the B^T march (node rows per block + 1): FP64 12 (one
+// block per CU at 3 waves per SIMD, <= 168 VGPRs), FP32 8 (two blocks per CU)
+template <typename T>
+constexpr int grad_march_waves()
+{
+    return sizeof(T) == 8 ? 12 : 8;
+}
 
 template <typename T>
 struct MassArgs {
