@@ -106,7 +106,7 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int n = a.n, nr = a.nr, ncr = nr - 1, ncl = n - 1;
     const int nxb = (n + OWN - 1) / OWN, nys = (n + a.seg - 1) / a.seg;
-    int b = blockIdx.x;
+    int b = xcd_tile(blockIdx.x, gridDim.x);
     const int ys = b % nys;
     b /= nys;
     const int xb = b % nxb;

@@ -7,6 +7,19 @@ namespace tn {
 
 constexpr int MARCH_SEG_MAX = 64; // node columns per block segment, at most
 
+// XCD-aware tile order.  The dispatcher hands workgroup b to XCD b % 8, and
+// each XCD has its own L2.  Tile t (segments fastest, then row blocks, then
+// diamonds) is given to workgroup b such that every XCD runs one contiguous
+// run of tiles: neighbouring tiles, which share halo rows and columns, then
+// share an L2.  A bijection of [0, nblocks).
+__device__ __forceinline__ int xcd_tile(int b, int nblocks)
+{
+    constexpr int XCDS = 8;
+    const int xcd = b % XCDS, local = b / XCDS;
+    const int per = nblocks / XCDS, rem = nblocks % XCDS;
+    return xcd * per + min(xcd, rem) + local;
+}
+
 template <typename T>
 struct MarchArgs {
     const T* src;             // [3][N]

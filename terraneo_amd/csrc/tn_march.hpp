@@ -70,7 +70,7 @@ __device__ __forceinline__ void run(Op& op, int n, int nr, int seg)
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int ncr = nr - 1;
     const int nxb = (n + OWN - 1) / OWN, nys = (n + seg - 1) / seg;
-    int b = blockIdx.x;
+    int b = xcd_tile(blockIdx.x, gridDim.x);
     const int ys = b % nys;
     b /= nys;
     const int xb = b % nxb;
