@@ -45,10 +45,13 @@ __host__ __device__ inline int blocks_for(int n, int seg, int waves)
 
 // Point-to-point hand-off between neighbouring waves through LDS counters
 // instead of a block barrier per step: the waves of a SIMD drift apart and hide
-// each other's memory latency.  The wait is bounded, so it always ends.
+// each other's memory latency.  The waves of a workgroup are co-resident and
+// every wave posts both counters at every step, so each wait is satisfied
+// after at most RING - 1 steps of a neighbour; the bound (about 40 ms at
+// 2.4 GHz) only guarantees that a wait ends whatever happens.
 __device__ __forceinline__ void wait_counter(int* c, int target)
 {
-    for (int i = 0; i < (1 << 16); ++i) {
+    for (int i = 0; i < (1 << 20); ++i) {
         if (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return;
         __builtin_amdgcn_s_sleep(1);
     }
