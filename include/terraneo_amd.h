@@ -240,6 +240,19 @@ int tn_boundary_assign(const tn_mesh* m, int dtype, int ncomp, void* dst, const 
 /* strong_algebraic_freeslip_enforcement_in_place (strong_algebraic_freeslip_enforcement.hpp:21-37) */
 int tn_freeslip_project(const tn_mesh* m, int dtype, void* v, int which, tn_stream_t s);
 
+/* ------------------------------------------------------------------ app caller (mantle circulation)
+ * Stokes right-hand side of one time step of the mantle-circulation app
+ * (mantlecirculation.cpp:876-898 with RHSVelocityInterpolator :117-145):
+ * f_u = M (Ra T n), n the unit position vector, then the velocity boundary
+ * conditions of K's viscous block enforced strongly (Dirichlet shells zero,
+ * free-slip shells without normal component); f_p = 0.  T[10][n][n][nr] on the
+ * velocity mesh, in K's dtype. */
+int tn_stokes_rhs_from_temperature(tn_stokes* K, const void* temperature, double rayleigh, void* f_u, void* f_p,
+                                   tn_stream_t s);
+/* pressure "normalisation" after each Stokes solve (mantlecirculation.cpp:926-932):
+ * p -= (sum of p over owned nodes) / (number of owned nodes) */
+int tn_pressure_normalize(const tn_mesh* m, int dtype, void* p, tn_stream_t s);
+
 /* ------------------------------------------------------------------ solvers
  * Multigrid with Chebyshev smoothing over `nlevels` operators ordered coarse
  * to fine (Multigrid multigrid.hpp:24-212, Chebyshev chebyshev.hpp:50-177).

@@ -94,6 +94,8 @@ SIGNATURES = {
     "tn_restrict": (i32, [vp, vp, i32, i32, vp, vp, vp]),
     "tn_boundary_assign": (i32, [vp, i32, i32, vp, vp, i32, vp]),
     "tn_freeslip_project": (i32, [vp, i32, vp, i32, vp]),
+    "tn_stokes_rhs_from_temperature": (i32, [vp, vp, f64, vp, vp, vp]),
+    "tn_pressure_normalize": (i32, [vp, i32, vp, vp]),
     "tn_gmg_create": (i32, [i32, pvp, vp, vp, pvp]),
     "tn_gmg_destroy": (i32, [vp]),
     "tn_gmg_solve": (i32, [vp, vp, vp, vp]),

@@ -223,6 +223,20 @@ def freeslip_project(mesh, v, which, stream=None):
     check(lib().tn_freeslip_project(mesh.h, dtype_code(v.dtype), _ptr(v), which, _stream_ptr(stream)))
 
 
+def stokes_rhs_from_temperature(K, temperature, rayleigh, fu, fp, stream=None):
+    """Stokes right-hand side of one mantle-circulation time step (mantlecirculation.cpp:876-898):
+    fu = M (Ra T n) with K's velocity BCs enforced, fp = 0."""
+    check(lib().tn_stokes_rhs_from_temperature(K.h, _ptr(temperature), float(rayleigh), _ptr(fu), _ptr(fp),
+                                               _stream_ptr(stream)))
+    return fu, fp
+
+
+def pressure_normalize(mesh, p, stream=None):
+    """p -= owned mean of p (mantlecirculation.cpp:926-932)."""
+    check(lib().tn_pressure_normalize(mesh.h, dtype_code(p.dtype), _ptr(p), _stream_ptr(stream)))
+    return p
+
+
 class Stokes:
     """EpsDivDivStokes drop-in: [A B^T; B 0] on (velocity mesh, pressure mesh)."""
 

@@ -113,6 +113,21 @@ __global__ __launch_bounds__(256) void k_dot_owned(const T* __restrict__ a, cons
     block_accumulate(s, out);
 }
 
+// out += sum over the owned entries of a one-component field (column walk)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sum_owned(const T* __restrict__ a, const uint8_t* __restrict__ owned_lat,
+                                                   int64_t N, int nr, double* out)
+{
+    double s = 0.0;
+    const int lane = threadIdx.x & 63;
+    for (ColumnWalk cw(N / nr, 1); cw.col < cw.ncol; cw.col += cw.step) {
+        if (!owned_lat[cw.col]) continue;
+        const T* ac = a + cw.col * nr;
+        for (int r = lane; r < nr; r += 64) s += (double)ac[r];
+    }
+    block_accumulate(s, out);
+}
+
 // One Chebyshev step fused (chebyshev.hpp:100-168 does it in four passes):
 // z = dinv (b - Az); d = alpha z + beta d; x += d.  az holds A x on entry and
 // is not written.  FIRST: beta = 0 and d is not read (it may hold anything).
@@ -429,6 +444,14 @@ void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, dou
 }
 
 template <typename T>
+void launch_sum_owned(const MeshLevel& m, const T* a, double* out_dev, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum_owned<T>, dim3(grid_for(m.N / m.nr, 4, 2048)), dim3(256), 0, s, a, m.d_owned_lat.p, m.N,
+                       m.nr, out_dev);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
 void launch_cheby_update(T* x, T* d, const T* az, const T* b, const T* dinv, int64_t count, double alpha, double beta,
                          bool first, hipStream_t s)
 {
@@ -559,6 +582,7 @@ template void launch_cast<float, double>(float*, const double*, int64_t, hipStre
     template void launch_lincomb<T>(int64_t, T*, int, const double*, const T* const*, double, hipStream_t);     \
     template void launch_dot_owned<T>(const MeshLevel&, int, const T*, const T*, double*, hipStream_t);          \
     template void launch_sub_scaled<T>(T*, int64_t, const double*, double, hipStream_t);                         \
+    template void launch_sum_owned<T>(const MeshLevel&, const T*, double*, hipStream_t);                          \
     template void launch_axpy_dot<T>(const MeshLevel&, int, T*, double, const T*, const T*, double*, hipStream_t); \
     template void launch_cheby_update<T>(T*, T*, const T*, const T*, const T*, int64_t, double, double, bool,   \
                                          hipStream_t);                                                           \

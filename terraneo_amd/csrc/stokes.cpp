@@ -117,6 +117,21 @@ __global__ void k_freeslip_project(T* v, const double* coords, int64_t N, int nr
     v[2 * N + i] = (T)((double)v[2 * N + i] - un * n2);
 }
 
+// RHSVelocityInterpolator (mantlecirculation.cpp:117-145): the nodal
+// buoyancy force Ra T n at every node copy, n the unit position vector
+template <typename T>
+__global__ void k_buoyancy(T* f, const T* temp, const double* coords, int64_t N, int nr, double rayleigh)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double* c = coords + 3 * (i / nr);
+    const double inv = 1.0 / sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    const double a = rayleigh * (double)temp[i] * inv;
+    f[i] = (T)(a * c[0]);
+    f[N + i] = (T)(a * c[1]);
+    f[2 * N + i] = (T)(a * c[2]);
+}
+
 }  // namespace
 
 namespace tn {
@@ -267,6 +282,70 @@ int tn_boundary_assign(const tn_mesh* m, int dtype, int ncomp, void* dst, const 
         hipLaunchKernelGGL(k_boundary_assign<float>, dim3(g), dim3(256), 0, (hipStream_t)s, (float*)dst,
                            (const float*)src, L.N, L.nr, ncomp, which);
     TN_HIP(hipGetLastError());
+    TN_API_END
+}
+
+int tn_stokes_rhs_from_temperature(tn_stokes* K, const void* temperature, double rayleigh, void* f_u, void* f_p,
+                                   tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(K && temperature && f_u && f_p, "null pointer");
+    TN_REQUIRE((const void*)f_u != temperature, "rhs: f_u must not alias the temperature");
+    auto& L = const_cast<tn_mesh*>(K->vel)->lvl.dev();
+    auto& P = const_cast<tn_mesh*>(K->pre)->lvl.dev();
+    const int bc[2] = {K->A->bc_cmb, K->A->bc_surf};
+    auto go = [&](auto* fu) {
+        using T = std::remove_pointer_t<decltype(fu)>;
+        const hipStream_t st = (hipStream_t)s;
+        const size_t bytes = sizeof(T) * 3 * (size_t)L.N;
+        if (K->rhs_tmp.n != bytes) K->rhs_tmp.alloc(bytes);
+        T* g = (T*)K->rhs_tmp.p;
+        hipLaunchKernelGGL(k_buoyancy<T>, dim3((unsigned)((L.N + 255) / 256)), dim3(256), 0, st, g,
+                           (const T*)temperature, L.d_coords.p, L.N, L.nr, rayleigh);
+        TN_HIP(hipGetLastError());
+        // f_u = M g, additive communication (vector_mass.hpp:59-153)
+        tn::launch_fill<T>(fu, 3 * L.N, T(0), st);
+        tn::MassArgs<T> a{L.d_coords.p, L.d_radii.p, L.n, L.nr, 3, g, nullptr, fu};
+        tn::launch_mass<T>(a, false, st);
+        tn::launch_interface_sum<T>(L, 3, fu, st);
+        // strong enforcement of the velocity BCs on the right-hand side
+        // (mantlecirculation.cpp:890-898): homogeneous Dirichlet rows zero,
+        // free-slip normal components removed
+        for (int shell = 0; shell < 2; ++shell) {
+            const int which = shell == 0 ? 1 : 2;
+            if (bc[shell] == TN_DIRICHLET) {
+                hipLaunchKernelGGL(k_boundary_assign<T>, dim3((unsigned)((L.N + 255) / 256)), dim3(256), 0, st, fu,
+                                   (const T*)nullptr, L.N, L.nr, 3, which);
+                TN_HIP(hipGetLastError());
+            } else if (bc[shell] == TN_FREESLIP) {
+                tn::launch_freeslip_project<T>(L, fu, which, st);
+            }
+        }
+        tn::launch_fill<T>((T*)f_p, P.N, T(0), st);
+    };
+    if (K->A->dtype == TN_F64)
+        go((double*)f_u);
+    else
+        go((float*)f_u);
+    TN_API_END
+}
+
+int tn_pressure_normalize(const tn_mesh* m, int dtype, void* p, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(m && p, "null pointer");
+    auto& L = const_cast<tn_mesh*>(m)->lvl.dev();
+    const hipStream_t st = (hipStream_t)s;
+    double* sum = L.d_scratch.p;
+    tn::launch_fill<double>(sum, 1, 0.0, st);
+    const double inv_owned = 1.0 / (double)L.owned;
+    if (dtype == TN_F64) {
+        tn::launch_sum_owned<double>(L, (const double*)p, sum, st);
+        tn::launch_sub_scaled<double>((double*)p, L.N, sum, inv_owned, st);
+    } else {
+        tn::launch_sum_owned<float>(L, (const float*)p, sum, st);
+        tn::launch_sub_scaled<float>((float*)p, L.N, sum, inv_owned, st);
+    }
     TN_API_END
 }
 

@@ -147,6 +147,9 @@ void launch_cheby_update(T* x, T* d, const T* az, const T* b, const T* dinv, int
 // y[i] -= scale * sum_dev[0], i < count
 template <typename T>
 void launch_sub_scaled(T* y, int64_t count, const double* sum_dev, double scale, hipStream_t s);
+// out_dev[0] += sum of a one-component field over owned nodes
+template <typename T>
+void launch_sum_owned(const MeshLevel& m, const T* a, double* out_dev, hipStream_t s);
 
 // fs/fs penalty: beta[0..3) = sum_owned r x u  ;  y (+)= gamma x r with gamma = S beta
 template <typename T>

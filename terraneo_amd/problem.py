@@ -165,7 +165,7 @@ class StokesProblem:
 
     # ---------------------------------------------------------------- solve
     def solve(self, bu, bp, method="fgmres", rtol=1e-8, max_iterations=500, restart=30, u=None, p=None,
-              pressure_mean_free=False):
+              pressure_mean_free=False, atol=1e-30):
         """pressure_mean_free: solve with the pressure constrained to zero mean
         (see tn_krylov_options.pressure_mean_free)."""
         torch = _torch()
@@ -173,7 +173,7 @@ class StokesProblem:
         p = torch.zeros(self.pre.N, **self.f64) if p is None else p
         meth = TN_FGMRES_BLOCKTRI if method == "fgmres" else TN_MINRES_BLOCKDIAG
         info = api.stokes_solve(self.K, meth, self.gmg, self.schur, u, p, bu, bp, max_iterations=max_iterations,
-                                rtol=rtol, atol=1e-30, restart=restart, pressure_mean_free=pressure_mean_free)
+                                rtol=rtol, atol=atol, restart=restart, pressure_mean_free=pressure_mean_free)
         return u, p, info
 
     def residual(self, u, p, bu, bp, pressure_mean_free=False):

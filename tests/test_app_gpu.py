@@ -1,0 +1,99 @@
+"""The app caller (apps/mantlecirculation/mantlecirculation.cpp:866-932) on the
+GPU: the Stokes right-hand side from temperature and the pressure
+normalisation against numpy restatements over the oracle's vector mass, and
+warm-started consecutive solves of the app driver.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from fields import consistent_random
+
+NEUMANN, DIRICHLET, FREESLIP = 0, 1, 2
+
+
+def _rhs_reference(ov, temp, rayleigh, bc):
+    """mantlecirculation.cpp:876-898: g = Ra T x/|x| per node (RHSVelocityInterpolator
+    :117-145), f = M g, then the Dirichlet shells zeroed
+    (strong_algebraic_homogeneous_velocity_dirichlet_enforcement_stokes_like) and the
+    normal component removed on free-slip shells (strong_algebraic_freeslip_enforcement)."""
+    xyz = ov.node_xyz().reshape(-1, 3)
+    nrm = xyz / np.linalg.norm(xyz, axis=1, keepdims=True)
+    g = np.ascontiguousarray((rayleigh * temp[:, None] * nrm).T.reshape(-1))
+    f = O.vector_mass(ov, g).reshape(3, -1)
+    r = np.arange(ov.N) % ov.nr
+    for shell, mode in ((r == 0, bc[0]), (r == ov.nr - 1, bc[1])):
+        if mode == DIRICHLET:
+            f[:, shell] = 0.0
+        elif mode == FREESLIP:
+            n = nrm[shell].T
+            f[:, shell] -= (f[:, shell] * n).sum(0) * n
+    return f.reshape(-1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bc", [(DIRICHLET, DIRICHLET), (FREESLIP, FREESLIP), (FREESLIP, DIRICHLET),
+                                (NEUMANN, NEUMANN)])
+def test_rhs_from_temperature_matches_reference(cuda, bc):
+    import torch
+
+    import terraneo_amd as tn
+
+    lat, ra = 3, 1e5
+    mv, mp = tn.Mesh(lat, lat, 0.5, 1.0), tn.Mesh(lat - 1, lat - 1, 0.5, 1.0)
+    ov = O.Mesh(lat, mv.nr, 0.5, 1.0)
+    eta = torch.ones(mv.N, dtype=torch.float64, device=cuda)
+    K = tn.Stokes(mv, mp, tn.Eps(mv, eta, *bc))
+    temp = consistent_random(ov, 1, 7).reshape(-1)
+    fu = torch.full((3 * mv.N,), 3.0, dtype=torch.float64, device=cuda)
+    fp = torch.full((mp.N,), 3.0, dtype=torch.float64, device=cuda)
+    tn.api.stokes_rhs_from_temperature(K, torch.from_numpy(temp).to(cuda), ra, fu, fp)
+    ref = _rhs_reference(ov, temp, ra, bc)
+    assert np.abs(fu.cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert torch.count_nonzero(fp).item() == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_pressure_normalize(cuda, dt):
+    import torch
+
+    import terraneo_amd as tn
+
+    mp = tn.Mesh(3, 3, 0.5, 1.0)
+    op = O.Mesh(3, mp.nr, 0.5, 1.0)
+    p = consistent_random(op, 1, 3).reshape(-1) + 0.25
+    own = op.owned.astype(bool)
+    ref = p - p[own].sum() / own.sum()
+    tdt = torch.float64 if dt == "f64" else torch.float32
+    pd = torch.from_numpy(p).to(cuda, tdt)
+    tn.api.pressure_normalize(mp, pd)
+    tol = 1e-14 if dt == "f64" else 2e-6
+    assert np.abs(pd.double().cpu().numpy() - ref).max() <= tol * np.abs(ref).max()
+    assert abs(pd.double().cpu().numpy()[own].sum()) <= tol * np.abs(p).sum()
+
+
+@pytest.mark.gpu
+def test_app_warm_started_steps(cuda):
+    """As in the app, each step runs a few FGMRES iterations from the previous
+    step's solution, so over steps with the same temperature the residual keeps
+    falling: each step starts where the last one stopped."""
+    import torch
+
+    from terraneo_amd.app import AppParameters, MantleCirculationStokes
+
+    prm = AppParameters(lat=3, rad=3, levels=3, krylov_max_iterations=5, krylov_rtol=1e-14, krylov_atol=1e-30)
+    app = MantleCirculationStokes(prm, device=cuda)
+    T = app.initial_temperature()
+    rows = [app.step(T) for _ in range(3)]
+    for prev, cur in zip(rows, rows[1:]):
+        # warm start: the new solve starts at the previous final residual (the
+        # pressure shift of the normalisation is a null mode with Dirichlet shells)
+        assert cur["initial_residual"] <= prev["final_residual"] * (1 + 1e-6) + 1e-14 * rows[0]["initial_residual"]
+        assert cur["final_residual"] < cur["initial_residual"]
+    assert rows[-1]["final_residual"] < 1e-3 * rows[0]["initial_residual"]
+    # the pressure is normalised after every step
+    pm = app.pr.pre
+    ones = torch.ones_like(app.p)
+    assert abs(pm.dot(app.p, ones, 1)) <= 1e-10 * np.sqrt(pm.dot(app.p, app.p, 1) * pm.owned_nodes)
+    assert len(app.table) == 3 and "iterations" in app.table_text()
