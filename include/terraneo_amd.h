@@ -234,6 +234,18 @@ int tn_prolongate_add(const tn_mesh* coarse, const tn_mesh* fine, int dtype, int
 /* RestrictionVecConstant (restriction_constant.hpp:134-250): xc = R xf */
 int tn_restrict(const tn_mesh* fine, const tn_mesh* coarse, int dtype, int ncomp, const void* xf, void* xc,
                 tn_stream_t s);
+/* ProlongationLinear / ProlongationVecLinear (prolongation_linear.hpp:16-191 / 193-370; weights
+ * grid_transfer_linear.hpp:13-157): geometry-aware linear interpolation, exact for affine fields.
+ * xf = P xc (apply_mode TN_REPLACE) or xf += P xc (TN_ADD); ncomp 1 (scalar) or 3 (vector).
+ * vec_early_return = 1 reproduces ProlongationVecLinear's return inside its component loop
+ * (:284,298): at fine nodes that coincide with or lie radially between coarse nodes only
+ * component 0 is transferred. */
+int tn_prolongate_linear(const tn_mesh* coarse, const tn_mesh* fine, int dtype, int ncomp, const void* xc, void* xf,
+                         int apply_mode, int vec_early_return, tn_stream_t s);
+/* RestrictionLinear / RestrictionVecLinear (restriction_linear.hpp:19-251 / 253-449): xc = P^T xf over owned
+ * fine nodes, then additive communication.  vec_early_return as above (:354,367). */
+int tn_restrict_linear(const tn_mesh* fine, const tn_mesh* coarse, int dtype, int ncomp, const void* xf, void* xc,
+                       int vec_early_return, tn_stream_t s);
 /* assign_masked_else_keep_old on shells: which = 1 (CMB) | 2 (SURFACE);
  * src NULL assigns zero (strong_algebraic_dirichlet_enforcement.hpp:88-123) */
 int tn_boundary_assign(const tn_mesh* m, int dtype, int ncomp, void* dst, const void* src, int which, tn_stream_t s);

@@ -214,6 +214,20 @@ def restrict(fine, coarse, xf, xc, ncomp=3, stream=None):
     return xc
 
 
+def prolongate_linear(coarse, fine, xc, xf, ncomp=3, add=False, vec_early_return=False, stream=None):
+    """Geometry-aware linear prolongation (ProlongationLinear / ProlongationVecLinear)."""
+    check(lib().tn_prolongate_linear(coarse.h, fine.h, dtype_code(xc.dtype), ncomp, _ptr(xc), _ptr(xf),
+                                     TN_ADD if add else TN_REPLACE, int(bool(vec_early_return)), _stream_ptr(stream)))
+    return xf
+
+
+def restrict_linear(fine, coarse, xf, xc, ncomp=3, vec_early_return=False, stream=None):
+    """Transpose of the linear prolongation over owned fine nodes (RestrictionLinear / RestrictionVecLinear)."""
+    check(lib().tn_restrict_linear(fine.h, coarse.h, dtype_code(xf.dtype), ncomp, _ptr(xf), _ptr(xc),
+                                   int(bool(vec_early_return)), _stream_ptr(stream)))
+    return xc
+
+
 def boundary_assign(mesh, dst, src, which, ncomp=3, stream=None):
     check(lib().tn_boundary_assign(mesh.h, dtype_code(dst.dtype), ncomp, _ptr(dst), _ptr(src), which,
                                    _stream_ptr(stream)))

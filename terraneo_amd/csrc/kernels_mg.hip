@@ -102,6 +102,166 @@ __global__ void k_prolongate_add(const T* __restrict__ xc, T* __restrict__ xf, i
     }
 }
 
+// ---------------------------------------------------------------- linear transfers
+// Geometry-aware linear prolongation (ProlongationLinear / ProlongationVecLinear,
+// prolongation_linear.hpp; weights grid_transfer_linear.hpp:13-157): a fine
+// node coincident with a coarse node takes its value; a fine node between two
+// radially aligned coarse nodes interpolates linearly in |x|; a lateral
+// midpoint interpolates between the chord midpoints of its coarse edge on the
+// bottom and top coarse shells (negative weights near the surface are part of
+// the scheme).  Positions are unit lateral coordinates times radii of the fine
+// mesh, in FP64.
+struct LinStencil {
+    int n;          // 1, 2 or 4 coarse nodes
+    int kase;       // 0 coincident, 1 radially aligned, 2 lateral midpoint
+    int64_t c[4];   // coarse node index (lat * nrc + r)
+    double w[4];
+};
+
+__device__ inline double lin_norm_at(const double* __restrict__ cf, const double* __restrict__ radf, int d, int nf,
+                                     int x, int y, int r)
+{
+    const double* u = cf + 3 * (((int64_t)d * nf + x) * nf + y);
+    const double R = radf[r];
+    const double p0 = u[0] * R, p1 = u[1] * R, p2 = u[2] * R;
+    return sqrt(p0 * p0 + p1 * p1 + p2 * p2);
+}
+
+__device__ inline double lin_chord_norm(const double* __restrict__ cf, const double* __restrict__ radf, int d,
+                                        int nf, int x0, int y0, int x1, int y1, int r)
+{
+    const double* u0 = cf + 3 * (((int64_t)d * nf + x0) * nf + y0);
+    const double* u1 = cf + 3 * (((int64_t)d * nf + x1) * nf + y1);
+    const double R = radf[r];
+    const double p0 = 0.5 * (u0[0] * R + u1[0] * R), p1 = 0.5 * (u0[1] * R + u1[1] * R),
+                 p2 = 0.5 * (u0[2] * R + u1[2] * R);
+    return sqrt(p0 * p0 + p1 * p1 + p2 * p2);
+}
+
+__device__ inline LinStencil lin_stencil(const double* __restrict__ cf, const double* __restrict__ radf, int d, int xf,
+                                         int yf, int rf, int nf, int nrf, int nc, int nrc)
+{
+    LinStencil s;
+    auto cidx = [&](int x, int y, int r) { return (((int64_t)d * nc + x) * nc + y) * nrc + r; };
+    if (!(xf & 1) && !(yf & 1) && !(rf & 1)) {
+        s.n = 1;
+        s.kase = 0;
+        s.c[0] = cidx(xf >> 1, yf >> 1, rf >> 1);
+        s.w[0] = 1.0;
+        return s;
+    }
+    const int rb = rf < nrf - 1 ? rf / 2 : rf / 2 - 1, rt = rb + 1;
+    const double fn = lin_norm_at(cf, radf, d, nf, xf, yf, rf);
+    if (!(xf & 1) && !(yf & 1)) {
+        const int X = xf >> 1, Y = yf >> 1;
+        const double bn = lin_norm_at(cf, radf, d, nf, 2 * X, 2 * Y, 2 * rb);
+        const double tn = lin_norm_at(cf, radf, d, nf, 2 * X, 2 * Y, 2 * rt);
+        const double nu = (fn - bn) / (tn - bn);
+        s.n = 2;
+        s.kase = 1;
+        s.c[0] = cidx(X, Y, rb);
+        s.c[1] = cidx(X, Y, rt);
+        s.w[0] = 1.0 - nu;
+        s.w[1] = nu;
+        return s;
+    }
+    int x0, x1, y0, y1;
+    if (!(xf & 1)) {  // "vertical" edge
+        x0 = x1 = xf / 2;
+        y0 = yf / 2;
+        y1 = yf / 2 + 1;
+    } else if (!(yf & 1)) {  // "horizontal" edge
+        x0 = xf / 2;
+        x1 = xf / 2 + 1;
+        y0 = y1 = yf / 2;
+    } else {  // "diagonal" edge
+        x0 = xf / 2 + 1;
+        x1 = xf / 2;
+        y0 = yf / 2;
+        y1 = yf / 2 + 1;
+    }
+    const double bn = lin_chord_norm(cf, radf, d, nf, 2 * x0, 2 * y0, 2 * x1, 2 * y1, 2 * rb);
+    const double tn = lin_chord_norm(cf, radf, d, nf, 2 * x0, 2 * y0, 2 * x1, 2 * y1, 2 * rt);
+    const double nu = (fn - bn) / (tn - bn);
+    s.n = 4;
+    s.kase = 2;
+    s.c[0] = cidx(x0, y0, rb);
+    s.c[1] = cidx(x1, y1, rb);
+    s.c[2] = cidx(x0, y0, rt);
+    s.c[3] = cidx(x1, y1, rt);
+    s.w[0] = s.w[1] = 0.5 * (1.0 - nu);
+    s.w[2] = s.w[3] = 0.5 * nu;
+    return s;
+}
+
+// xf (+)= P_lin xc, thread per fine node.  quirk: ProlongationVecLinear's
+// early return (prolongation_linear.hpp:284,298) -- at coincident and radially
+// aligned fine nodes only component 0 is transferred.
+template <typename T>
+__global__ void k_prolongate_linear(const T* __restrict__ xc, T* __restrict__ xf, const double* __restrict__ cf,
+                                    const double* __restrict__ radf, int nf, int nrf, int nc, int nrc, int64_t Nf,
+                                    int64_t Nc, int ncomp, int add, int quirk)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= Nf) return;
+    const int rf = (int)(tid % nrf);
+    int64_t lat = tid / nrf;
+    const int yf = (int)(lat % nf);
+    lat /= nf;
+    const int xf_ = (int)(lat % nf);
+    const int d = (int)(lat / nf);
+    const LinStencil st = lin_stencil(cf, radf, d, xf_, yf, rf, nf, nrf, nc, nrc);
+    const int nc_used = (quirk && st.kase < 2) ? 1 : ncomp;
+    for (int c = 0; c < ncomp; ++c) {
+        double v = 0.0;
+        if (c < nc_used)
+            for (int k = 0; k < st.n; ++k) v += st.w[k] * (double)xc[c * Nc + st.c[k]];
+        if (add)
+            xf[c * Nf + tid] += (T)v;
+        else
+            xf[c * Nf + tid] = (T)v;
+    }
+}
+
+// xc = R_lin xf (the transpose over owned fine nodes; interface sum follows):
+// thread per coarse node, gathering from the fine nodes whose stencil holds
+// it (lateral offsets -1..1, radial -2..2), no atomics.  quirk: the early
+// return of RestrictionVecLinear (restriction_linear.hpp:354,367).
+template <typename T>
+__global__ void k_restrict_linear(const T* __restrict__ xf, T* __restrict__ xc, const uint8_t* __restrict__ owned_lat_f,
+                                  const double* __restrict__ cf, const double* __restrict__ radf, int nf, int nrf,
+                                  int nc, int nrc, int64_t Nf, int64_t Nc, int ncomp, int quirk)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= Nc) return;
+    const int R = (int)(tid % nrc);
+    int64_t lat = tid / nrc;
+    const int Y = (int)(lat % nc);
+    lat /= nc;
+    const int X = (int)(lat % nc);
+    const int d = (int)(lat / nc);
+    double acc[3] = {0, 0, 0};
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int fx = 2 * X + dx, fy = 2 * Y + dy;
+            if (fx < 0 || fx >= nf || fy < 0 || fy >= nf) continue;
+            const int64_t flat = ((int64_t)d * nf + fx) * nf + fy;
+            if (!owned_lat_f[flat]) continue;
+            for (int dr = -2; dr <= 2; ++dr) {
+                const int fr = 2 * R + dr;
+                if (fr < 0 || fr >= nrf) continue;
+                const LinStencil st = lin_stencil(cf, radf, d, fx, fy, fr, nf, nrf, nc, nrc);
+                double w = 0.0;
+                for (int k = 0; k < st.n; ++k)
+                    if (st.c[k] == tid) w += st.w[k];
+                if (w == 0.0) continue;
+                const int nc_used = (quirk && st.kase < 2) ? 1 : ncomp;
+                for (int c = 0; c < nc_used; ++c) acc[c] += w * (double)xf[c * Nf + flat * nrf + fr];
+            }
+        }
+    for (int c = 0; c < ncomp; ++c) xc[c * Nc + tid] = (T)acc[c];
+}
+
 __device__ inline double pc_lat_weight(int dx, int dy)
 {
     if (dx == 0 && dy == 0) return 1.0;
@@ -625,6 +785,27 @@ void launch_prolongate_add(const MeshLevel& mc, const MeshLevel& mf, int ncomp, 
 }
 
 template <typename T>
+void launch_prolongate_linear(const MeshLevel& mc, const MeshLevel& mf, int ncomp, const T* xc, T* xf, bool add,
+                              bool quirk, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_prolongate_linear<T>, dim3((unsigned)((mf.N + 255) / 256)), dim3(256), 0, s, xc, xf,
+                       mf.d_coords.p, mf.d_radii.p, mf.n, mf.nr, mc.n, mc.nr, mf.N, mc.N, ncomp, add ? 1 : 0,
+                       quirk ? 1 : 0);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_restrict_linear(const MeshLevel& mf, const MeshLevel& mc, int ncomp, const T* xf, T* xc, bool quirk,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(k_restrict_linear<T>, dim3((unsigned)((mc.N + 255) / 256)), dim3(256), 0, s, xf, xc,
+                       mf.d_owned_lat.p, mf.d_coords.p, mf.d_radii.p, mf.n, mf.nr, mc.n, mc.nr, mf.N, mc.N, ncomp,
+                       quirk ? 1 : 0);
+    TN_HIP(hipGetLastError());
+    launch_interface_sum<T>(mc, ncomp, xc, s);
+}
+
+template <typename T>
 void launch_restrict(const MeshLevel& mf, const MeshLevel& mc, int ncomp, const T* xf, T* xc, hipStream_t s)
 {
     hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((mc.N + 255) / 256)), dim3(256), 0, s, xf, xc,
@@ -699,7 +880,10 @@ int64_t stored_cellsum_elems(int n, int nr)
 
 #define TN_INST_T(T)                                                                                           \
     template void launch_prolongate_add<T>(const MeshLevel&, const MeshLevel&, int, const T*, T*, hipStream_t); \
-    template void launch_restrict<T>(const MeshLevel&, const MeshLevel&, int, const T*, T*, hipStream_t);
+    template void launch_restrict<T>(const MeshLevel&, const MeshLevel&, int, const T*, T*, hipStream_t);       \
+    template void launch_prolongate_linear<T>(const MeshLevel&, const MeshLevel&, int, const T*, T*, bool, bool,  \
+                                              hipStream_t);                                                     \
+    template void launch_restrict_linear<T>(const MeshLevel&, const MeshLevel&, int, const T*, T*, bool, hipStream_t);
 TN_INST_T(double)
 TN_INST_T(float)
 #define TN_INST_GCA(T, S, SF) template void launch_gca_build<T, S, SF>(const GcaArgs<T, S, SF>&, bool, hipStream_t);

@@ -269,6 +269,41 @@ int tn_restrict(const tn_mesh* fine, const tn_mesh* coarse, int dtype, int ncomp
     TN_API_END
 }
 
+int tn_prolongate_linear(const tn_mesh* coarse, const tn_mesh* fine, int dtype, int ncomp, const void* xc, void* xf,
+                         int apply_mode, int vec_early_return, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(coarse && fine && xc && xf, "null pointer");
+    TN_REQUIRE(ncomp == 1 || ncomp == 3, "linear prolongation: ncomp must be 1 or 3");
+    auto& mc = const_cast<tn_mesh*>(coarse)->lvl.dev();
+    auto& mf = const_cast<tn_mesh*>(fine)->lvl.dev();
+    TN_REQUIRE(2 * (mc.n - 1) == mf.n - 1 && 2 * (mc.nr - 1) == mf.nr - 1, "meshes are not a refinement pair");
+    const bool add = apply_mode == TN_ADD, quirk = vec_early_return != 0 && ncomp == 3;
+    if (dtype == TN_F64)
+        tn::launch_prolongate_linear<double>(mc, mf, ncomp, (const double*)xc, (double*)xf, add, quirk,
+                                             (hipStream_t)s);
+    else
+        tn::launch_prolongate_linear<float>(mc, mf, ncomp, (const float*)xc, (float*)xf, add, quirk, (hipStream_t)s);
+    TN_API_END
+}
+
+int tn_restrict_linear(const tn_mesh* fine, const tn_mesh* coarse, int dtype, int ncomp, const void* xf, void* xc,
+                       int vec_early_return, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(coarse && fine && xc && xf, "null pointer");
+    TN_REQUIRE(ncomp == 1 || ncomp == 3, "linear restriction: ncomp must be 1 or 3");
+    auto& mc = const_cast<tn_mesh*>(coarse)->lvl.dev();
+    auto& mf = const_cast<tn_mesh*>(fine)->lvl.dev();
+    TN_REQUIRE(2 * (mc.n - 1) == mf.n - 1 && 2 * (mc.nr - 1) == mf.nr - 1, "meshes are not a refinement pair");
+    const bool quirk = vec_early_return != 0 && ncomp == 3;
+    if (dtype == TN_F64)
+        tn::launch_restrict_linear<double>(mf, mc, ncomp, (const double*)xf, (double*)xc, quirk, (hipStream_t)s);
+    else
+        tn::launch_restrict_linear<float>(mf, mc, ncomp, (const float*)xf, (float*)xc, quirk, (hipStream_t)s);
+    TN_API_END
+}
+
 int tn_boundary_assign(const tn_mesh* m, int dtype, int ncomp, void* dst, const void* src, int which, tn_stream_t s)
 {
     TN_API_BEGIN

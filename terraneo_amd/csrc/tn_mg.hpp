@@ -101,6 +101,13 @@ template <typename T>
 void launch_prolongate_add(const MeshLevel& mc, const MeshLevel& mf, int ncomp, const T* xc, T* xf, hipStream_t s);
 template <typename T>
 void launch_restrict(const MeshLevel& mf, const MeshLevel& mc, int ncomp, const T* xf, T* xc, hipStream_t s);
+// geometry-aware linear transfers (quirk: the Vec variants' early return)
+template <typename T>
+void launch_prolongate_linear(const MeshLevel& mc, const MeshLevel& mf, int ncomp, const T* xc, T* xf, bool add,
+                              bool quirk, hipStream_t s);
+template <typename T>
+void launch_restrict_linear(const MeshLevel& mf, const MeshLevel& mc, int ncomp, const T* xf, T* xc, bool quirk,
+                            hipStream_t s);
 template <typename T, typename S, typename SF>
 void launch_gca_build(const GcaArgs<T, S, SF>& a, bool fine_geo, hipStream_t s);
 template <typename T, typename S>

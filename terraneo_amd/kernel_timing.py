@@ -14,6 +14,10 @@
         and then with the FP32 V-cycle.  A small torch sort launch brackets
         each timed solve, so a kernel trace can be cut into per-iteration
         kernel costs
+    python -m terraneo_amd.kernel_timing transfers [--lat 7] [--reps 20]
+        vector prolongation (Add) and restriction between lateral levels lat-1
+        and lat (radial 2^lat + 1 fine nodes): the constant transfers and the
+        geometry-aware linear ones
     python -m terraneo_amd.kernel_timing eps [--lat 7] [--reps 20]
         the matrix-free epsilon-div-div apply (Replace, no interface sum) at
         lateral level lat x (2^lat + 1) radial nodes for each boundary-condition
@@ -133,13 +137,25 @@ def solve(lat, reps):
         torch.cuda.empty_cache()
 
 
+def transfers(lat, reps):
+    dev = torch.device("cuda:0")
+    mf, mc = api.Mesh(lat, lat, 0.55, 1.0), api.Mesh(lat - 1, lat - 1, 0.55, 1.0)
+    xf = torch.randn(3 * mf.N, dtype=torch.float64, device=dev)
+    xc = torch.randn(3 * mc.N, dtype=torch.float64, device=dev)
+    for name, fn in [("constant P (Add)", lambda: api.prolongate_add(mc, mf, xc, xf)),
+                     ("constant R", lambda: api.restrict(mf, mc, xf, xc)),
+                     ("linear P (Add)", lambda: api.prolongate_linear(mc, mf, xc, xf, add=True)),
+                     ("linear R", lambda: api.restrict_linear(mf, mc, xf, xc))]:
+        print(f"velocity level {lat}: {name}: {_time(fn, reps):.3f} ms per call", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["stored", "coupling", "eps", "solve"])
+    ap.add_argument("what", choices=["stored", "coupling", "eps", "solve", "transfers"])
     ap.add_argument("--lat", type=int, default=7)
     ap.add_argument("--reps", type=int, default=10)
     a = ap.parse_args()
-    {"stored": stored, "coupling": coupling, "eps": eps, "solve": solve}[a.what](a.lat, a.reps)
+    {"stored": stored, "coupling": coupling, "eps": eps, "solve": solve, "transfers": transfers}[a.what](a.lat, a.reps)
 
 
 if __name__ == "__main__":
