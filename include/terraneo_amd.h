@@ -195,6 +195,16 @@ int tn_eps_set_penalty_epsilon(tn_eps* op, double eps);
 int tn_eps_gca_from(tn_eps* coarse, const tn_eps* fine, int store_dtype, tn_stream_t s);
 /* same with an explicit radial bracket (tn_gca_bracket); tn_eps_gca_from == TN_GCA_REFERENCE */
 int tn_eps_gca_from_bracket(tn_eps* coarse, const tn_eps* fine, int store_dtype, int bracket, tn_stream_t s);
+/* GCAElementsCollector (gca_elements_collector.hpp:32-96): flags (a scalar field on `coarsest`, dtype `dtype`)
+ * is set to 1 at the coarsest-level cells (x, y, r) that have a descendant cell on `fine` with a wedge where the
+ * reference-coordinate gradient of the nodal coefficient k exceeds 10 at the centroid, and to 0 elsewhere. */
+int tn_gca_collect_elements(const tn_mesh* fine, const tn_mesh* coarsest, int dtype, const void* k, void* flags,
+                            tn_stream_t s);
+/* Selective stored-matrix mode (OperatorStoredMatrixMode::Selective, local_matrix_storage.hpp:68-100): after
+ * tn_eps_gca_from, the cells of `coarse` whose coarsest ancestor is not flagged use the rediscretized wedge
+ * matrices of `coarse` (its geometry and nodal viscosity, as the reference's on-the-fly assembly for cells
+ * without a stored matrix); flagged cells keep their Galerkin matrices.  flags in `coarse`'s dtype. */
+int tn_eps_gca_select(tn_eps* coarse, const tn_mesh* coarsest, const void* flags, tn_stream_t s);
 /* get_local_matrix of a stored operator (kerngen.hpp:481-495), host out[18*18],
  * dimension-wise dof order (component*6 + wedge node); synchronous */
 int tn_eps_get_local_matrix(const tn_eps* op, int sd, int x, int y, int r, int wedge, double* out);

@@ -92,6 +92,8 @@ SIGNATURES = {
     "tn_kmass_lumped": (i32, [vp, i32, vp, vp, vp]),
     "tn_prolongate_add": (i32, [vp, vp, i32, i32, vp, vp, vp]),
     "tn_restrict": (i32, [vp, vp, i32, i32, vp, vp, vp]),
+    "tn_gca_collect_elements": (i32, [vp, vp, i32, vp, vp, vp]),
+    "tn_eps_gca_select": (i32, [vp, vp, vp, vp]),
     "tn_prolongate_linear": (i32, [vp, vp, i32, i32, vp, vp, i32, i32, vp]),
     "tn_restrict_linear": (i32, [vp, vp, i32, i32, vp, vp, i32, vp]),
     "tn_boundary_assign": (i32, [vp, i32, i32, vp, vp, i32, vp]),

@@ -183,6 +183,10 @@ class Eps:
         b = {"reference": 0, "geometric": 1}[bracket]
         check(lib().tn_eps_gca_from_bracket(self.h, fine.h, store_dtype, b, _stream_ptr(stream)))
 
+    def gca_select(self, coarsest, flags, stream=None):
+        """Selective GCA: rediscretized matrices on cells whose coarsest ancestor is not flagged."""
+        check(lib().tn_eps_gca_select(self.h, coarsest.h, _ptr(flags), _stream_ptr(stream)))
+
     def local_matrix(self, sd, x, y, r, w):
         out = np.zeros(324)
         check(lib().tn_eps_get_local_matrix(self.h, sd, x, y, r, w, out.ctypes.data))
@@ -212,6 +216,14 @@ def prolongate_add(coarse, fine, xc, xf, ncomp=3, stream=None):
 def restrict(fine, coarse, xf, xc, ncomp=3, stream=None):
     check(lib().tn_restrict(fine.h, coarse.h, dtype_code(xf.dtype), ncomp, _ptr(xf), _ptr(xc), _stream_ptr(stream)))
     return xc
+
+
+def gca_collect_elements(fine, coarsest, k, flags, stream=None):
+    """GCAElementsCollector: flags (scalar field on `coarsest`) = 1 where a descendant fine cell has
+    |grad k| > 10 (reference coordinates), else 0."""
+    check(lib().tn_gca_collect_elements(fine.h, coarsest.h, dtype_code(k.dtype), _ptr(k), _ptr(flags),
+                                        _stream_ptr(stream)))
+    return flags
 
 
 def prolongate_linear(coarse, fine, xc, xf, ncomp=3, add=False, vec_early_return=False, stream=None):

@@ -594,6 +594,56 @@ int tn_eps_gca_from_bracket(tn_eps* coarse, const tn_eps* fine, int store_dtype,
     TN_API_END
 }
 
+namespace {
+// levels between a mesh and the coarsest mesh of its hierarchy (both directions refined alike)
+int levels_between(const MeshLevel& m, const MeshLevel& coarsest)
+{
+    for (int L = 0; L < 20; ++L)
+        if ((int64_t)(coarsest.n - 1) << L == m.n - 1 && (int64_t)(coarsest.nr - 1) << L == m.nr - 1) return L;
+    TN_REQUIRE(false, "selective GCA: the mesh is not a uniform refinement of the coarsest mesh");
+    return -1;
+}
+}  // namespace
+
+int tn_gca_collect_elements(const tn_mesh* fine, const tn_mesh* coarsest, int dtype, const void* k, void* flags,
+                            tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(fine && coarsest && k && flags, "null pointer");
+    auto& mf = const_cast<tn_mesh*>(fine)->lvl.dev();
+    auto& m0 = const_cast<tn_mesh*>(coarsest)->lvl.dev();
+    const int L = levels_between(mf, m0);
+    if (dtype == TN_F64)
+        tn::launch_gca_collect<double>(mf, m0, L, (const double*)k, (double*)flags, (hipStream_t)s);
+    else
+        tn::launch_gca_collect<float>(mf, m0, L, (const float*)k, (float*)flags, (hipStream_t)s);
+    TN_API_END
+}
+
+int tn_eps_gca_select(tn_eps* coarse, const tn_mesh* coarsest, const void* flags, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(coarse && coarsest && flags, "null pointer");
+    TN_REQUIRE(coarse->stored_dtype >= 0, "selective GCA: the operator has no Galerkin matrices (tn_eps_gca_from first)");
+    TN_REQUIRE(coarse->eta && !coarse->eta_cell, "selective GCA: the operator needs a nodal viscosity of its level");
+    auto& m = const_cast<tn_mesh*>(coarse->mesh)->lvl.dev();
+    auto& m0 = const_cast<tn_mesh*>(coarsest)->lvl.dev();
+    const int L = levels_between(m, m0);
+    auto go = [&](auto* flg, auto* eta) {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(flg)>>;
+        if (coarse->stored_dtype == TN_F64)
+            tn::launch_gca_select<T, double>(m0, L, flg, m, eta, (double*)coarse->stored.p, (hipStream_t)s);
+        else
+            tn::launch_gca_select<T, float>(m0, L, flg, m, eta, (float*)coarse->stored.p, (hipStream_t)s);
+    };
+    if (coarse->dtype == TN_F64)
+        go((const double*)flags, (const double*)coarse->eta);
+    else
+        go((const float*)flags, (const float*)coarse->eta);
+    coarse->blk_ready = false;  // boundary forms follow the new matrices
+    TN_API_END
+}
+
 int tn_eps_get_local_matrix(const tn_eps* op, int sd, int x, int y, int r, int wedge, double* out)
 {
     TN_API_BEGIN

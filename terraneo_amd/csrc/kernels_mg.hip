@@ -462,6 +462,80 @@ __global__ void k_gca_build(GcaArgs<T, S, SF> a)
         }
 }
 
+// ---------------------------------------------------------------- selective GCA
+// GCAElementsCollector (gca_elements_collector.hpp:32-96): a fine cell whose
+// wedge has |grad k| > 10 at the centroid, the gradient taken in reference
+// coordinates (grad_shape of integrands.hpp at the one-point Felippa point
+// (1/3, 1/3, 0)), marks its ancestor cell on the coarsest level (indices
+// shifted right by `levels`).  Every writer stores the same 1: the race is benign.
+template <typename T>
+__global__ void k_gca_collect(const T* __restrict__ k, int n, int nr, int levels, T* __restrict__ flags, int n0,
+                              int nr0)
+{
+    const int ncl = n - 1, ncr = nr - 1;
+    const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= 10LL * ncl * ncl * ncr) return;
+    const int r = (int)(cell % ncr);
+    int64_t lat = cell / ncr;
+    const int y = (int)(lat % ncl);
+    lat /= ncl;
+    const int x = (int)(lat % ncl);
+    const int d = (int)(lat / ncl);
+    bool mark = false;
+    for (int w = 0; w < 2; ++w) {
+        double kv[6];
+        for (int j = 0; j < 6; ++j)
+            kv[j] = (double)k[(((int64_t)d * n + x + WOFF[w][j][0]) * n + y + WOFF[w][j][1]) * nr + r + WOFF[w][j][2]];
+        // grad_shape at (1/3, 1/3, 0): lateral parts times shape_rad = 1/2, radial
+        // part shape_lat = 1/3 times grad_shape_rad = -+1/2
+        const double gx = 0.5 * ((kv[1] - kv[0]) + (kv[4] - kv[3]));
+        const double gy = 0.5 * ((kv[2] - kv[0]) + (kv[5] - kv[3]));
+        const double gz = (1.0 / 6.0) * ((kv[3] + kv[4] + kv[5]) - (kv[0] + kv[1] + kv[2]));
+        if (sqrt(gx * gx + gy * gy + gz * gz) > 10.0) mark = true;
+    }
+    if (mark)
+        flags[(((int64_t)d * n0 + (x >> levels)) * n0 + (y >> levels)) * nr0 + (r >> levels)] = T(1);
+}
+
+// Selective storage (local_matrix_storage.hpp:68-100 with the operator's
+// on-the-fly assembly for cells without a stored matrix): the cells of a
+// Galerkin level whose coarsest ancestor is not flagged get the level's own
+// rediscretized wedge matrices (geometry and nodal viscosity of the level).
+// One thread per (cell, wedge, component block q <= p), as k_gca_build.
+template <typename T, typename S>
+__global__ void k_gca_select(const T* __restrict__ flags, int n0, int nr0, int levels, const double* __restrict__ geo,
+                             const double* __restrict__ radii, const T* __restrict__ eta, int n, int nr,
+                             S* __restrict__ stored)
+{
+    constexpr int QP[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    const int ncl = n - 1, ncr = nr - 1;
+    const int64_t ncells = 10LL * ncl * ncl * ncr;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= ncells * 12) return;
+    const int blk = (int)(tid % 6);
+    const int w = (int)((tid / 6) % 2);
+    const int64_t cell = tid / 12;
+    const int r = (int)(cell % ncr);
+    int64_t lat = cell / ncr;
+    const int y = (int)(lat % ncl);
+    lat /= ncl;
+    const int x = (int)(lat % ncl);
+    const int d = (int)(lat / ncl);
+    if (flags[(((int64_t)d * n0 + (x >> levels)) * n0 + (y >> levels)) * nr0 + (r >> levels)] != T(0)) return;
+    const int q = QP[blk][0], p = QP[blk][1];
+    const double* geo_w = geo + ((((int64_t)d * ncl + x) * ncl + y) * 2 + w) * GEO_STRIDE;
+    double ksum = 0.0;
+    for (int j = 0; j < 6; ++j)
+        ksum += (double)eta[(((int64_t)d * n + x + WOFF[w][j][0]) * n + y + WOFF[w][j][1]) * nr + r + WOFF[w][j][2]];
+    double A[6][6];
+    eps_block_geo<double>(geo_w, radii[r], radii[r + 1], ksum, q, p, A);
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            const int rr = i + 6 * q, cc = j + 6 * p;
+            if (rr <= cc) stored[stored_at<S>(cell, w, pk(rr, cc))] = (S)A[i][j];
+        }
+}
+
 // ---------------------------------------------------------------- stored-matrix apply
 //
 // Two launches, every node copy finished once, no atomics, deterministic:
@@ -814,6 +888,27 @@ void launch_restrict(const MeshLevel& mf, const MeshLevel& mc, int ncomp, const 
     launch_interface_sum<T>(mc, ncomp, xc, s);
 }
 
+template <typename T>
+void launch_gca_collect(const MeshLevel& fine, const MeshLevel& coarsest, int levels, const T* k, T* flags,
+                        hipStream_t s)
+{
+    launch_fill<T>(flags, coarsest.N, T(0), s);
+    const int64_t cells = 10LL * (fine.n - 1) * (fine.n - 1) * (fine.nr - 1);
+    hipLaunchKernelGGL(k_gca_collect<T>, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, k, fine.n, fine.nr,
+                       levels, flags, coarsest.n, coarsest.nr);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename T, typename S>
+void launch_gca_select(const MeshLevel& coarsest, int levels, const T* flags, const MeshLevel& m, const T* eta,
+                       S* stored, hipStream_t s)
+{
+    const int64_t total = 12LL * 10 * (m.n - 1) * (m.n - 1) * (m.nr - 1);
+    hipLaunchKernelGGL((k_gca_select<T, S>), dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, flags,
+                       coarsest.n, coarsest.nr, levels, m.d_geo.p, m.d_radii.p, eta, m.n, m.nr, stored);
+    TN_HIP(hipGetLastError());
+}
+
 template <typename T, typename S, typename SF>
 void launch_gca_build(const GcaArgs<T, S, SF>& a, bool fine_geo, hipStream_t s)
 {
@@ -886,6 +981,17 @@ int64_t stored_cellsum_elems(int n, int nr)
     template void launch_restrict_linear<T>(const MeshLevel&, const MeshLevel&, int, const T*, T*, bool, hipStream_t);
 TN_INST_T(double)
 TN_INST_T(float)
+template void launch_gca_collect<double>(const MeshLevel&, const MeshLevel&, int, const double*, double*,
+                                         hipStream_t);
+template void launch_gca_collect<float>(const MeshLevel&, const MeshLevel&, int, const float*, float*, hipStream_t);
+template void launch_gca_select<double, double>(const MeshLevel&, int, const double*, const MeshLevel&,
+                                                const double*, double*, hipStream_t);
+template void launch_gca_select<double, float>(const MeshLevel&, int, const double*, const MeshLevel&,
+                                               const double*, float*, hipStream_t);
+template void launch_gca_select<float, double>(const MeshLevel&, int, const float*, const MeshLevel&, const float*,
+                                               double*, hipStream_t);
+template void launch_gca_select<float, float>(const MeshLevel&, int, const float*, const MeshLevel&, const float*,
+                                              float*, hipStream_t);
 #define TN_INST_GCA(T, S, SF) template void launch_gca_build<T, S, SF>(const GcaArgs<T, S, SF>&, bool, hipStream_t);
 TN_INST_GCA(double, double, double)
 TN_INST_GCA(double, double, float)

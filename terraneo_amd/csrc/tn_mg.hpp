@@ -110,6 +110,14 @@ void launch_restrict_linear(const MeshLevel& mf, const MeshLevel& mc, int ncomp,
                             hipStream_t s);
 template <typename T, typename S, typename SF>
 void launch_gca_build(const GcaArgs<T, S, SF>& a, bool fine_geo, hipStream_t s);
+// selective GCA: flags on the coarsest mesh (zeroed, then marked) from the fine nodal coefficient k
+template <typename T>
+void launch_gca_collect(const MeshLevel& fine, const MeshLevel& coarsest, int levels, const T* k, T* flags,
+                        hipStream_t s);
+// rediscretized matrices for the cells of m whose coarsest ancestor is not flagged
+template <typename T, typename S>
+void launch_gca_select(const MeshLevel& coarsest, int levels, const T* flags, const MeshLevel& m, const T* eta,
+                       S* stored, hipStream_t s);
 template <typename T, typename S>
 void launch_stored_apply(const StoredArgs<T, S>& a, bool diagonal, hipStream_t s);
 // elements of StoredArgs::csum + ctop for an n x n x nr mesh
