@@ -23,7 +23,10 @@ levels 1..4, r in [0.5, 1], no-slip shells, Ra = 1e5, FGMRES(10) with at most 10
 iterations per step to rel. 1e-6 / abs. 1e-12, one V-cycle with Chebyshev order
 2 and 2 smoothing steps, 10 power iterations, coarsest-level PCG(50, 1e-6, 1e-16).
 
-    python -m terraneo_amd.app [--lat 4] [--steps 5] [--free-slip] [--prec f32]
+    python -m terraneo_amd.app [--lat 4] [--steps 5] [--free-slip] [--prec f32] [--xdmf DIR]
+
+With --xdmf, every step also writes the velocity and the temperature as an XDMF
+time series / checkpoint (terraneo_amd.xdmf), as the app's output block does.
 """
 import argparse
 import time
@@ -130,6 +133,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--free-slip", action="store_true", help="free-slip shells (mean-free pressure)")
     ap.add_argument("--prec", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--xdmf", default=None, help="write an XDMF time series / checkpoint to this directory")
     a = ap.parse_args()
     torch = _torch()
     prm = AppParameters(lat=a.lat, rad=a.lat, levels=a.levels, prec_dtype=a.prec)
@@ -140,9 +144,19 @@ def main():
     T = app.initial_temperature()
     # a temperature that changes a little per step, as the energy step would
     lat = torch.from_numpy(np.asarray(app.pr.vel.lateral_coords()).reshape(-1, 3)).to(T.device)
+    out, Tk = None, T
+    if a.xdmf:
+        from .xdmf import XDMFOutput
+
+        out = XDMFOutput(a.xdmf, app.pr.vel)
+        out.add("velocity", lambda: app.u, vec_dim=3)
+        out.add("temperature", lambda: Tk)
     for k in range(a.steps):
         wobble = 0.02 * torch.sin(3.0 * lat[:, 0] + 0.3 * k)[:, None].expand(-1, app.pr.vel.nr).reshape(-1)
-        row = app.step(T * (1.0 + wobble))
+        Tk = T * (1.0 + wobble)
+        row = app.step(Tk)
+        if out is not None:
+            out.write()
         print(f"timestep {row['timestep']}: {row['iterations']} FGMRES iterations, "
               f"residual {row['initial_residual']:.3e} -> {row['final_residual']:.3e}, "
               f"{row['step_s'] * 1e3:.1f} ms", flush=True)
