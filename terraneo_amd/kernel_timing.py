@@ -24,6 +24,13 @@
         pair, FP64 and FP32; free-slip on both shells includes the rotation
         penalty (its moment pass and the fused store)
 
+    python -m terraneo_amd.kernel_timing slice TRACE.csv [--reps 10]
+        cuts the rocprofv3 kernel trace (CSV) of a `solve` run at the
+        bracketing sort launches and prints, per precision, the kernel time
+        and calls per FGMRES iteration, the trace span against the kernel-busy
+        time, and the idle time before each kernel grouped by the kernel
+        before it (host-side gaps)
+
 Each line reports the mean wall time per call over `reps` calls after three
 warm-up calls (stream-synchronised), so that a `rocprofv3 --kernel-trace
 --stats` run of the same command attributes the time to kernels.
@@ -149,12 +156,58 @@ def transfers(lat, reps):
         print(f"velocity level {lat}: {name}: {_time(fn, reps):.3f} ms per call", flush=True)
 
 
+def slice_trace(path, reps):
+    """Per-iteration kernel costs of the two solves in a kernel trace of `solve`."""
+    import collections
+    import csv
+
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    marks = [i for i, r in enumerate(rows) if "sort" in r["Kernel_Name"].lower()]
+    clusters = []
+    for i in marks:
+        if not clusters or i - clusters[-1][-1] > 3:
+            clusters.append([i])
+        else:
+            clusters[-1].append(i)
+    if len(clusters) < 4:
+        raise SystemExit("trace: expected two bracketed solves (4 sort launches)")
+    spans = [(clusters[0][-1], clusters[1][0]), (clusters[2][-1], clusters[3][0])]
+    for name, (a, b) in zip(("f64", "mixed"), spans):
+        seg = rows[a + 1:b]
+        t0, t1 = int(seg[0]["Start_Timestamp"]), int(seg[-1]["End_Timestamp"])
+        tot = collections.defaultdict(lambda: [0, 0.0])
+        gaps = collections.defaultdict(float)
+        busy = 0.0
+        for p, r in zip([None] + seg[:-1], seg):
+            dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+            k = r["Kernel_Name"][:100]
+            tot[k][0] += 1
+            tot[k][1] += dur
+            busy += dur
+            if p is not None:
+                g = (int(r["Start_Timestamp"]) - int(p["End_Timestamp"])) / 1e6
+                if g > 0:
+                    gaps[p["Kernel_Name"][:70]] += g
+        span = (t1 - t0) / 1e6
+        print(f"== {name}: {len(seg)} kernels, per iteration span {span / reps:.2f} ms, "
+              f"kernel-busy {busy / reps:.2f} ms")
+        for k, (c, d) in sorted(tot.items(), key=lambda kv: -kv[1][1])[:24]:
+            print(f"  {d / reps:8.3f} ms/it {c / reps:7.1f} calls/it {d / c * 1e3:9.1f} us  {k}")
+        print(f"  idle before a kernel, by the kernel before it (ms/it):")
+        for k, g in sorted(gaps.items(), key=lambda kv: -kv[1])[:4]:
+            print(f"  {g / reps:8.3f}  {k}")
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["stored", "coupling", "eps", "solve", "transfers"])
+    ap.add_argument("what", choices=["stored", "coupling", "eps", "solve", "transfers", "slice"])
+    ap.add_argument("trace", nargs="?", help="slice: rocprofv3 kernel trace CSV of a `solve` run")
     ap.add_argument("--lat", type=int, default=7)
     ap.add_argument("--reps", type=int, default=10)
     a = ap.parse_args()
+    if a.what == "slice":
+        slice_trace(a.trace, a.reps)
+        return
     {"stored": stored, "coupling": coupling, "eps": eps, "solve": solve, "transfers": transfers}[a.what](a.lat, a.reps)
 
 
