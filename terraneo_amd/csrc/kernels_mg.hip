@@ -48,6 +48,20 @@ __device__ __forceinline__ float vec_get(const float4& v, int i)
     return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
 }
 __device__ __forceinline__ double vec_get(const double2& v, int i) { return i == 0 ? v.x : v.y; }
+// The matrix stream is read once per apply and is larger than the L2 and the
+// last-level cache: non-temporal loads keep it from evicting the vectors.
+__device__ __forceinline__ float4 ld_stream(const float4* p)
+{
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 r = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+    return make_float4(r.x, r.y, r.z, r.w);
+}
+__device__ __forceinline__ double2 ld_stream(const double2* p)
+{
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    const v2 r = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return make_double2(r.x, r.y);
+}
 
 // A wave-uniform pointer, made opaque through readfirstlane so that a load
 // p[off32] becomes one global_load with an SGPR base and a 32-bit VGPR offset
@@ -541,8 +555,8 @@ __global__ void k_gca_select(const T* __restrict__ flags, int n0, int nr0, int l
 // Two launches, every node copy finished once, no atomics, deterministic:
 //  * k_stored_cells: a wave per lateral quad and chunk of 64 radial cells
 //    (one per lane).  Each cell applies its two wedge matrices (plain
-//    product, or their diagonal) read as coalesced 16-byte vectors of the tile
-//    layout (stored_at); the cell's 8 corner sums are merged radially with a
+//    product, or their diagonal) read as coalesced non-temporal 16-byte
+//    vectors of the tile layout (stored_at); the cell's 8 corner sums are merged radially with a
 //    lane shuffle (node level c = bottom of cell c + top of cell c-1) and the
 //    4 lateral corners x 3 components at level c are written per cell, the
 //    chunk's top level once per chunk.  No barriers: the matrix stream runs at
@@ -604,7 +618,7 @@ __device__ __forceinline__ void stored_cell_sums(const StoredArgs<T, S>& a, int6
             for (int i = 0; i < 18; ++i) {
 #pragma unroll
                 for (int j = i; j < 18; ++j, ++k) {
-                    if (k % V == 0) cur = vp[(k / V) * 64];
+                    if (k % V == 0) cur = ld_stream(vp + (k / V) * 64);
                     const T aij = (T)vec_get(cur, k % V);
                     Y(i) += aij * xs[j];
                     if (j != i) Y(j) += aij * xs[i];
