@@ -139,6 +139,23 @@ struct tn_gmg {
 };
 
 namespace {
+// A tn_gmg as a preconditioner: one multigrid solve on the finest operator.
+struct GmgPrec : Solver {
+    tn_gmg* g = nullptr;
+    void solve(Ctx& c, LinOp&, const BVec& x, const BVec& b) override
+    {
+        g->ctx.s = c.s;
+        g->mg->solve(g->ctx, *g->lops.back(), x, b);
+    }
+    void solve_zero(Ctx& c, LinOp&, const BVec& x, const BVec& b) override
+    {
+        g->ctx.s = c.s;
+        g->mg->solve_zero(g->ctx, *g->lops.back(), x, b);
+    }
+};
+}  // namespace
+
+namespace {
 
 void gmg_setup(tn_gmg* g, const tn_gmg_options& o, hipStream_t s)
 {
@@ -313,14 +330,7 @@ int tn_pcg_solve(tn_eps* A, tn_gmg* prec, void* x, const void* b, const tn_krylo
     for (int i = 0; i < 4; ++i) t.push_back(st.make(X));
     KrylovParams kp{o->max_iterations, o->rtol, o->atol, o->restart};
     EpsOp op(A);
-    struct GmgPrec : Solver {
-        tn_gmg* g;
-        void solve(Ctx& c, LinOp&, const BVec& x, const BVec& b) override
-        {
-            g->ctx.s = c.s;
-            g->mg->solve(g->ctx, *g->lops.back(), x, b);
-        }
-    } gp;
+    GmgPrec gp;
     gp.g = prec;
     PCG pcg(kp, t, prec ? &gp : nullptr);
     TN_HIP(hipStreamSynchronize(c.s));
@@ -359,14 +369,7 @@ int tn_stokes_solve(tn_stokes* K, int method, tn_gmg* prec_u, const void* schur_
     StokesOp Kop(K);
     NullOp null;
     // velocity block preconditioner, possibly in another precision
-    struct GmgPrec : Solver {
-        tn_gmg* g;
-        void solve(Ctx& c, LinOp&, const BVec& x, const BVec& b) override
-        {
-            g->ctx.s = c.s;
-            g->mg->solve(g->ctx, *g->lops.back(), x, b);
-        }
-    } gp;
+    GmgPrec gp;
     gp.g = prec_u;
     std::unique_ptr<PrecisionBridge> bridge;
     Solver* pu = &gp;

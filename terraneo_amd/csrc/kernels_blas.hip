@@ -130,17 +130,23 @@ __global__ __launch_bounds__(256) void k_sum_owned(const T* __restrict__ a, cons
 
 // One Chebyshev step fused (chebyshev.hpp:100-168 does it in four passes):
 // z = dinv (b - Az); d = alpha z + beta d; x += d.  az holds A x on entry and
-// is not written.  FIRST: beta = 0 and d is not read (it may hold anything).
-template <typename T, bool FIRST>
+// is not written.  MODE 1 (first step): beta = 0 and d is not read (it may
+// hold anything).  MODE 2 (first step from x = 0): also A x = 0, so neither
+// az nor x is read and x = d is written -- the values of MODE 1 on a zeroed
+// x, up to the sign of zero.
+template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_cheby_update(T* __restrict__ x, T* __restrict__ d, const T* __restrict__ az,
                                                       const T* __restrict__ b, const T* __restrict__ dinv,
                                                       int64_t count, T alpha, T beta)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
-        const T z = dinv[i] * (b[i] - az[i]);
-        const T dn = FIRST ? alpha * z : alpha * z + beta * d[i];
+        const T z = MODE == 2 ? dinv[i] * b[i] : dinv[i] * (b[i] - az[i]);
+        const T dn = MODE == 0 ? alpha * z + beta * d[i] : alpha * z;
         d[i] = dn;
-        x[i] += dn;
+        if constexpr (MODE == 2)
+            x[i] = dn;
+        else
+            x[i] += dn;
     }
 }
 
@@ -453,13 +459,15 @@ void launch_sum_owned(const MeshLevel& m, const T* a, double* out_dev, hipStream
 
 template <typename T>
 void launch_cheby_update(T* x, T* d, const T* az, const T* b, const T* dinv, int64_t count, double alpha, double beta,
-                         bool first, hipStream_t s)
+                         int mode, hipStream_t s)
 {
     const dim3 gr(grid_for(count)), bl(256);
-    if (first)
-        hipLaunchKernelGGL((k_cheby_update<T, true>), gr, bl, 0, s, x, d, az, b, dinv, count, (T)alpha, (T)beta);
+    if (mode == 2)
+        hipLaunchKernelGGL((k_cheby_update<T, 2>), gr, bl, 0, s, x, d, az, b, dinv, count, (T)alpha, (T)beta);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_cheby_update<T, 1>), gr, bl, 0, s, x, d, az, b, dinv, count, (T)alpha, (T)beta);
     else
-        hipLaunchKernelGGL((k_cheby_update<T, false>), gr, bl, 0, s, x, d, az, b, dinv, count, (T)alpha, (T)beta);
+        hipLaunchKernelGGL((k_cheby_update<T, 0>), gr, bl, 0, s, x, d, az, b, dinv, count, (T)alpha, (T)beta);
     TN_HIP(hipGetLastError());
 }
 
@@ -584,7 +592,7 @@ template void launch_cast<float, double>(float*, const double*, int64_t, hipStre
     template void launch_sub_scaled<T>(T*, int64_t, const double*, double, hipStream_t);                         \
     template void launch_sum_owned<T>(const MeshLevel&, const T*, double*, hipStream_t);                          \
     template void launch_axpy_dot<T>(const MeshLevel&, int, T*, double, const T*, const T*, double*, hipStream_t); \
-    template void launch_cheby_update<T>(T*, T*, const T*, const T*, const T*, int64_t, double, double, bool,   \
+    template void launch_cheby_update<T>(T*, T*, const T*, const T*, const T*, int64_t, double, double, int,    \
                                          hipStream_t);                                                           \
     template void launch_interface_sum<T>(const MeshLevel&, int, T*, hipStream_t);                               \
     template void launch_penalty_moment<T>(const MeshLevel&, const T*, double*, double*, hipStream_t);                    \

@@ -214,18 +214,37 @@ void DiagonalSolver::solve(Ctx& c, LinOp&, const BVec& x, const BVec& b)
     v_mul(c, x, d_);
 }
 
-void BlockDiagonal::solve(Ctx& c, LinOp&, const BVec& x, const BVec& b)
+void Solver::solve_zero(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
+{
+    v_fill(c, x, 0.0);
+    solve(c, A, x, b);
+}
+
+void BlockDiagonal::solve(Ctx& c, LinOp&, const BVec& x, const BVec& b) { run(c, x, b, false); }
+
+void BlockDiagonal::solve_zero(Ctx& c, LinOp&, const BVec& x, const BVec& b) { run(c, x, b, true); }
+
+void BlockDiagonal::run(Ctx& c, const BVec& x, const BVec& b, bool zero)
 {
     BVec xu, xp, bu, bp;
     xu.b[0] = x.b[0];
     xp.b[0] = x.b[1];
     bu.b[0] = b.b[0];
     bp.b[0] = b.b[1];
-    p11_->solve(c, *A11_, xu, bu);
-    p22_->solve(c, *A22_, xp, bp);
+    if (zero) {
+        p11_->solve_zero(c, *A11_, xu, bu);
+        p22_->solve_zero(c, *A22_, xp, bp);
+    } else {
+        p11_->solve(c, *A11_, xu, bu);
+        p22_->solve(c, *A22_, xp, bp);
+    }
 }
 
-void BlockTriangular::solve(Ctx& c, LinOp&, const BVec& x, const BVec& b)
+void BlockTriangular::solve(Ctx& c, LinOp&, const BVec& x, const BVec& b) { run(c, x, b, false); }
+
+void BlockTriangular::solve_zero(Ctx& c, LinOp&, const BVec& x, const BVec& b) { run(c, x, b, true); }
+
+void BlockTriangular::run(Ctx& c, const BVec& x, const BVec& b, bool zero)
 {
     BVec xu, xp, bu, bp, tu;
     xu.b[0] = x.b[0];
@@ -233,11 +252,17 @@ void BlockTriangular::solve(Ctx& c, LinOp&, const BVec& x, const BVec& b)
     bu.b[0] = b.b[0];
     bp.b[0] = b.b[1];
     tu.b[0] = tmp_.b[0];
-    p22_->solve(c, *A22_, xp, bp);       // x2 = S^-1 b2
+    if (zero)
+        p22_->solve_zero(c, *A22_, xp, bp);  // x2 = S^-1 b2
+    else
+        p22_->solve(c, *A22_, xp, bp);
     B12_->apply(c, xp, tu);              // t1 = B^T x2
     v_lincomb(c, tu, {1.0, 1.0}, {&tu, &bu});
     v_lincomb(c, xp, {-1.0}, {&xp});
-    p11_->solve(c, *A11_, xu, tu);       // x1 = A^-1 (b1 + B^T x2)
+    if (zero)
+        p11_->solve_zero(c, *A11_, xu, tu);  // x1 = A^-1 (b1 + B^T x2)
+    else
+        p11_->solve(c, *A11_, xu, tu);
 }
 
 void PrecisionBridge::solve(Ctx& c, LinOp&, const BVec& x, const BVec& b)
@@ -245,6 +270,13 @@ void PrecisionBridge::solve(Ctx& c, LinOp&, const BVec& x, const BVec& b)
     v_convert(c, xi_, x);
     v_convert(c, bi_, b);
     inner_->solve(c, *op_, xi_, bi_);
+    v_convert(c, x, xi_);
+}
+
+void PrecisionBridge::solve_zero(Ctx& c, LinOp&, const BVec& x, const BVec& b)
+{
+    v_convert(c, bi_, b);
+    inner_->solve_zero(c, *op_, xi_, bi_);
     v_convert(c, x, xi_);
 }
 
@@ -276,7 +308,11 @@ struct DiagScaled : LinOp {
 };
 }  // namespace
 
-void Chebyshev::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
+void Chebyshev::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) { run(c, A, x, b, false); }
+
+void Chebyshev::solve_zero(Ctx& c, LinOp& A, const BVec& x, const BVec& b) { run(c, A, x, b, true); }
+
+void Chebyshev::run(Ctx& c, LinOp& A, const BVec& x, const BVec& b, bool zero)
 {
     if (need_lambda_) {
         // power iteration on D^-1 A from a deterministic random start
@@ -302,38 +338,46 @@ void Chebyshev::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;  // carried across outer iterations, as the reference does
     // one step: z = D^-1 (b - A x); d = alpha z + beta d; x += d (fused into one
-    // pass over the vectors unless a custom scaling is set)
-    auto step = [&](double alpha, double beta, bool first) {
-        A.apply(c, x, z_);
+    // pass over the vectors unless a custom scaling is set).  from_zero (the
+    // first step of solve_zero): A x = 0 is not computed and x = d is written.
+    auto step = [&](double alpha, double beta, bool first, bool from_zero) {
+        if (!from_zero) A.apply(c, x, z_);
+        const int mode = from_zero ? 2 : (first ? 1 : 0);
         if (!scale) {
             for (int i = 0; i < x.nb; ++i) {
                 const Field& f = x.b[i];
                 if (f.dtype == TN_F64)
                     launch_cheby_update<double>((double*)f.p, (double*)d_.b[i].p, (const double*)z_.b[i].p,
                                                 (const double*)b.b[i].p, (const double*)dinv_.b[i].p, f.count(), alpha,
-                                                beta, first, c.s);
+                                                beta, mode, c.s);
                 else
                     launch_cheby_update<float>((float*)f.p, (float*)d_.b[i].p, (const float*)z_.b[i].p,
                                                (const float*)b.b[i].p, (const float*)dinv_.b[i].p, f.count(), alpha,
-                                               beta, first, c.s);
+                                               beta, mode, c.s);
             }
             return;
         }
-        v_lincomb(c, z_, {1.0, -1.0}, {&b, &z_});
+        if (from_zero)
+            v_assign(c, z_, b);
+        else
+            v_lincomb(c, z_, {1.0, -1.0}, {&b, &z_});
         apply_scale(c, z_);
         if (first)
             v_lincomb(c, d_, {alpha}, {&z_});
         else
             v_lincomb(c, d_, {alpha, beta}, {&z_, &d_});
-        v_lincomb(c, x, {1.0, 1.0}, {&x, &d_});
+        if (from_zero)
+            v_assign(c, x, d_);
+        else
+            v_lincomb(c, x, {1.0, 1.0}, {&x, &d_});
     };
     for (int it = 0; it < iters_; ++it) {
-        step(1.0 / theta, 0.0, true);
+        step(1.0 / theta, 0.0, true, zero && it == 0);
         for (int i = 2; i <= order_; ++i) {
             const double rho_new = 1.0 / (2.0 * sigma - rho);
             const double beta = rho_new * rho;
             const double alpha = 2.0 * rho_new / delta;
-            step(alpha, beta, false);
+            step(alpha, beta, false, false);
             rho = rho_new;
         }
     }
@@ -357,8 +401,7 @@ void PCG::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
     // (identity, Jacobi), for which that is immaterial; a multigrid V-cycle
     // starts from the content of its output, so z is cleared first -- the
     // convention PMINRES and FGMRES follow (pminres.hpp:130, fgmres.hpp:177).
-    if (prec_) v_fill(c, z, 0.0);
-    P.solve(c, A, z, r);
+    P.solve_zero(c, A, z, r);
     v_assign(c, pp, z);
     const double r0 = std::sqrt(v_dot(c, r, r));
     info.initial_residual = info.final_residual = r0;
@@ -375,8 +418,7 @@ void PCG::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
         info.final_residual = res;
         info.history.push_back(res);
         if (res / r0 <= p_.rtol || res < p_.atol) return;
-        if (prec_) v_fill(c, z, 0.0);
-        P.solve(c, A, z, r);
+        P.solve_zero(c, A, z, r);
         const double beta = v_dot(c, z, r) / alpha_num;
         v_lincomb(c, pp, {1.0, beta}, {&z, &pp});
     }
@@ -411,8 +453,7 @@ void PMINRES::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
         const double delta = v_dot(c, az, z);
         v_lincomb(c, vjm1, {1.0, -delta / gamma_j, -gamma_j / gamma_jm1}, {&az, &vj, &vjm1});
         std::swap(vjm1, vj);
-        v_fill(c, zjp1, 0.0);
-        P.solve(c, A, zjp1, vj);
+        P.solve_zero(c, A, zjp1, vj);
         const double gamma_jp1 = std::sqrt(v_dot(c, zjp1, vj));
         const double a0 = c_j * delta - c_jm1 * s_j * gamma_j;
         const double a1 = std::sqrt(a0 * a0 + gamma_jp1 * gamma_jp1);
@@ -475,8 +516,7 @@ void FGMRES::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
         g[0] = beta0;
         int inner = 0;
         for (int j = 0; j < m && total < p_.max_iterations; ++j, ++total) {
-            v_fill(c, Z(j), 0.0);
-            P.solve(c, A, Z(j), V(j));
+            P.solve_zero(c, A, Z(j), V(j));
             if (skip_nonfinite_ && v_nonfinite(c, Z(j))) {
                 // skip the preconditioner for this step (fgmres.hpp:180-193)
                 v_assign(c, Z(j), V(j));
@@ -574,16 +614,29 @@ void fine_project(const Transfer& t, const BVec& v, hipStream_t s)
 void Multigrid::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
 {
     const int L = (int)tr_.size();
-    for (int cyc = 0; cyc < cycles_; ++cyc) recurse(c, A, x, b, L);
+    for (int cyc = 0; cyc < cycles_; ++cyc) recurse(c, A, x, b, L, false);
 }
 
-void Multigrid::recurse(Ctx& c, LinOp& A, const BVec& x, const BVec& b, int level)
+void Multigrid::solve_zero(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
+{
+    const int L = (int)tr_.size();
+    for (int cyc = 0; cyc < cycles_; ++cyc) recurse(c, A, x, b, L, cyc == 0);
+}
+
+// zero: x is to be taken as zero on entry (its contents are ignored)
+void Multigrid::recurse(Ctx& c, LinOp& A, const BVec& x, const BVec& b, int level, bool zero)
 {
     if (level == 0) {
-        coarse_->solve(c, A, e_[0], r_[0]);
+        if (zero)
+            coarse_->solve_zero(c, A, x, b);
+        else
+            coarse_->solve(c, A, x, b);
         return;
     }
-    sm_[level]->solve(c, A, x, b);
+    if (zero)
+        sm_[level]->solve_zero(c, A, x, b);
+    else
+        sm_[level]->solve(c, A, x, b);
     A.apply(c, x, tmp_[level]);
     v_lincomb(c, tmp_[level], {1.0, -1.0}, {&b, &tmp_[level]});
     const Transfer& t = tr_[level - 1];
@@ -595,8 +648,8 @@ void Multigrid::recurse(Ctx& c, LinOp& A, const BVec& x, const BVec& b, int leve
         fine_project<float>(t, tmp_[level], c.s);
         do_restrict<float>(t, tmp_[level], r_[level - 1], c.s);
     }
-    v_fill(c, e_[level - 1], 0.0);
-    recurse(c, *ops_[level - 1], e_[level - 1], r_[level - 1], level - 1);
+    // the coarse correction starts from zero
+    recurse(c, *ops_[level - 1], e_[level - 1], r_[level - 1], level - 1, true);
     if (f64) {
         do_prolongate<double>(t, e_[level - 1], x, c.s);
         fine_project<double>(t, x, c.s);

@@ -139,10 +139,12 @@ void launch_freeslip_project(const MeshLevel& m, T* v, int which, hipStream_t s)
 template <typename T>
 void launch_axpy_dot(const MeshLevel& m, int ncomp, T* w, double alpha, const T* x, const T* y, double* out_dev,
                      hipStream_t s);
-// Chebyshev step: z = dinv (b - az); d = alpha z + beta d (first: alpha z); x += d
+// Chebyshev step: z = dinv (b - az); d = alpha z + beta d; x += d.
+// mode 0: general; 1: first step (d = alpha z); 2: first step from x = 0
+// (az and x not read, x = d written; az may be null)
 template <typename T>
 void launch_cheby_update(T* x, T* d, const T* az, const T* b, const T* dinv, int64_t count, double alpha, double beta,
-                         bool first, hipStream_t s);
+                         int mode, hipStream_t s);
 
 // y[i] -= scale * sum_dev[0], i < count
 template <typename T>

@@ -78,6 +78,11 @@ struct LinOp {
 struct Solver {
     // x is the initial guess on entry (reference semantics: solvers start from x)
     virtual void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) = 0;
+    // the same from the zero initial guess; x's contents on entry are ignored.
+    // Default: zero x, then solve.  Solvers that can skip work for a zero
+    // guess (the product A 0, the fill itself) override it with a result
+    // identical to the default's.
+    virtual void solve_zero(Ctx& c, LinOp& A, const BVec& x, const BVec& b);
     virtual ~Solver() = default;
 };
 
@@ -99,12 +104,15 @@ struct KrylovParams {
 class IdentitySolver : public Solver {
   public:
     void solve(Ctx& c, LinOp&, const BVec& x, const BVec& b) override { v_assign(c, x, b); }
+    // x is overwritten either way
+    void solve_zero(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override { solve(c, A, x, b); }
 };
 
 class DiagonalSolver : public Solver {  // x = b * inv_diag
   public:
     explicit DiagonalSolver(BVec inv_diag) : d_(inv_diag) {}
     void solve(Ctx& c, LinOp&, const BVec& x, const BVec& b) override;
+    void solve_zero(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override { solve(c, A, x, b); }
 
   private:
     BVec d_;
@@ -114,6 +122,8 @@ class Chebyshev : public Solver {
   public:
     Chebyshev(int order, int iterations, BVec inv_diag, BVec t0, BVec t1, int power_iterations, uint64_t seed);
     void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+    // first step without A x (= 0) and with x written, not updated
+    void solve_zero(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
     double lambda_max() const { return lambda_; }
     void set_lambda_max(double l)
     {
@@ -124,6 +134,7 @@ class Chebyshev : public Solver {
     std::function<void(Ctx&, const BVec&)> scale;
 
   private:
+    void run(Ctx& c, LinOp& A, const BVec& x, const BVec& b, bool zero);
     void apply_scale(Ctx& c, const BVec& z);
     int order_, iters_, power_its_;
     uint64_t seed_;
@@ -188,9 +199,12 @@ class Multigrid : public Solver {
     Multigrid(std::vector<LinOp*> coarse_ops, std::vector<Transfer> transfers, std::vector<Solver*> smoothers,
               Solver* coarse, std::vector<BVec> tmp, std::vector<BVec> r, std::vector<BVec> e, int cycles);
     void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+    // the first cycle's pre-smoothing starts from zero on every level, so the
+    // corrections need no fill and the smoothers skip their first product
+    void solve_zero(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
 
   private:
-    void recurse(Ctx& c, LinOp& A, const BVec& x, const BVec& b, int level);
+    void recurse(Ctx& c, LinOp& A, const BVec& x, const BVec& b, int level, bool zero);
     std::vector<LinOp*> ops_;
     std::vector<Transfer> tr_;
     std::vector<Solver*> sm_;
@@ -204,6 +218,10 @@ class BlockDiagonal : public Solver {
   public:
     BlockDiagonal(LinOp* A11, Solver* p11, LinOp* A22, Solver* p22) : A11_(A11), p11_(p11), A22_(A22), p22_(p22) {}
     void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+    void solve_zero(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+
+  private:
+    void run(Ctx& c, const BVec& x, const BVec& b, bool zero);
 
   private:
     LinOp *A11_;
@@ -219,8 +237,10 @@ class BlockTriangular : public Solver {
     {
     }
     void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+    void solve_zero(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
 
   private:
+    void run(Ctx& c, const BVec& x, const BVec& b, bool zero);
     LinOp* A11_;
     Solver* p11_;
     LinOp* A22_;
@@ -236,6 +256,8 @@ class PrecisionBridge : public Solver {
   public:
     PrecisionBridge(Solver* inner, LinOp* inner_op, BVec xi, BVec bi) : inner_(inner), op_(inner_op), xi_(xi), bi_(bi) {}
     void solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
+    // no conversion of x: the inner solver starts from zero
+    void solve_zero(Ctx& c, LinOp& A, const BVec& x, const BVec& b) override;
 
   private:
     Solver* inner_;
