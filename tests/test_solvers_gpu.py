@@ -88,3 +88,32 @@ def test_c0_pcg_gmg_gca_solves(cuda, store):
     assert ok == 0
     xu = O.to_unique(om, x.cpu().numpy(), 3)
     assert np.abs(xu - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bc", ["dirichlet", "freeslip"])
+def test_vcycle_zero_guess_path(cuda, bc):
+    """A preconditioner call (zero initial guess passed down: no A 0 product on
+    any level, corrections written rather than cleared and updated) gives the
+    V-cycle of an explicit zero initial guess.  One PCG step returns alpha z
+    with z the preconditioner output for r0 = b."""
+    import torch
+
+    import terraneo_amd as tn
+
+    mode = tn.TN_DIRICHLET if bc == "dirichlet" else tn.TN_FREESLIP
+    meshes = [tn.Mesh(l, r, 0.55, 1.0) for l, r in [(2, 2), (3, 3), (4, 4)]]
+    oms = [O.Mesh(l, m.nr, 0.55, 1.0) for l, m in zip((2, 3, 4), meshes)]
+    etas = [radial_eta(om.radii, (10, om.n, om.n, om.nr)).reshape(-1) for om in oms]
+    ops = [tn.Eps(m, torch.from_numpy(e).to(cuda), mode, mode) for m, e in zip(meshes, etas)]
+    ops[1].gca_from(ops[2], tn.TN_F64)
+    ops[0].gca_from(ops[1], tn.TN_F64)
+    mg = tn.Gmg(ops, tn.api.gmg_options(cheby_order=3, cheby_iterations=2, power_iterations=10, coarse_solver="dense"))
+    b = torch.from_numpy(_rhs(oms[-1], 7)).to(cuda)
+    z = torch.zeros(3 * meshes[-1].N, dtype=torch.float64, device=cuda)
+    mg.solve(z, b)  # explicit zero initial guess (also fixes the eigenvalue estimates)
+    x = torch.zeros_like(z)
+    tn.pcg_solve(ops[-1], x, b, prec=mg, max_iterations=1, rtol=1e-30)
+    zn, xn = z / z.norm(), x / x.norm()
+    assert float(z.norm()) > 0
+    assert float((xn - zn).abs().max()) <= 1e-13
