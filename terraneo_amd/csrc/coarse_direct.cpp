@@ -7,9 +7,9 @@
 // vectors -- so every boundary treatment and the free-slip penalty are
 // exactly those of the operator -- then inverted once with rocSOLVER
 // (LU with partial pivoting, getrf + getri).  A coarse solve is then one
-// gather of the owned values, one FP64 gemv with the inverse and one scatter
-// plus interface sum: a few microseconds at the usual coarse sizes instead of
-// hundreds of host-synchronised PCG iterations.
+// gather of the owned values, one FP64 product with the inverse
+// (launch_dense_gemv: one pass over the matrix) and one scatter plus
+// interface sum, instead of hundreds of host-synchronised PCG iterations.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -41,6 +41,7 @@ DenseCoarse::DenseCoarse(tn_eps* op, hipStream_t s) : op_(op)
     inv_.alloc((size_t)n_ * n_);
     xu_.alloc(n_);
     yu_.alloc(n_);
+    part_.alloc((size_t)dense_gemv_chunks((int)n_) * n_);
     rb_check(rocblas_create_handle(&handle_), "create_handle");
 
     // assemble: column j = A e_j restricted to the owned copies
@@ -100,11 +101,7 @@ void DenseCoarse::solve(Ctx& c, LinOp&, const BVec& x, const BVec& b)
         launch_gather_owned<double>((const double*)fb.p, m.N, owned_idx_.p, nown_, xu_.p, c.s);
     else
         launch_gather_owned<float>((const float*)fb.p, m.N, owned_idx_.p, nown_, xu_.p, c.s);
-    rb_check(rocblas_set_stream(handle_, c.s), "set_stream");
-    const double one = 1.0, zero = 0.0;
-    rb_check(rocblas_dgemv(handle_, rocblas_operation_none, (rocblas_int)n_, (rocblas_int)n_, &one, inv_.p,
-                           (rocblas_int)n_, xu_.p, 1, &zero, yu_.p, 1),
-             "dgemv");
+    launch_dense_gemv(inv_.p, xu_.p, (int)n_, part_.p, yu_.p, c.s);
     if (fx.dtype == TN_F64) {
         launch_fill<double>((double*)fx.p, 3 * m.N, 0.0, c.s);
         launch_scatter_owned<double>((double*)fx.p, m.N, owned_idx_.p, nown_, yu_.p, c.s);

@@ -6,6 +6,7 @@
 // (set/lincomb/masked_dot_product), communication/shell/communication.hpp:241-739
 // (additive interface reduction, single-process form), epsilon_divdiv_kerngen.hpp:287-408,583-607
 // (fs/fs penalty).
+#include "tn_coarse.hpp"
 #include "tn_internal.hpp"
 
 namespace tn {
@@ -658,6 +659,52 @@ template <typename T>
 void launch_set_owned_unit(T* v, int64_t N, const int64_t* owned_idx, int c, hipStream_t s)
 {
     hipLaunchKernelGGL(k_set_owned_unit<T>, dim3(1), dim3(64), 0, s, v, N, owned_idx, c);
+    TN_HIP(hipGetLastError());
+}
+namespace {
+// y = A x for the dense column-major n x n coarsest-level inverse, in two
+// passes.  Pass 1: a thread per row over a chunk of DENSE_GEMV_CHUNK columns
+// (the grid's y index) writes partial[chunk][row]; the matrix is streamed once
+// with non-temporal loads, x is read at wave-uniform addresses.  Pass 2 sums
+// the chunks of a row in a fixed order (deterministic, no atomics).
+__global__ __launch_bounds__(256) void k_dense_gemv_partial(const double* __restrict__ A, const double* __restrict__ x,
+                                                            int n, double* __restrict__ partial)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int j0 = blockIdx.y * DENSE_GEMV_CHUNK;
+    const int j1 = min(n, j0 + DENSE_GEMV_CHUNK);
+    const double* a = A + (int64_t)j0 * n + i;
+    double acc = 0.0;
+    int j = j0;
+    for (; j + 8 <= j1; j += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = __builtin_nontemporal_load(a + (int64_t)(j - j0 + k) * n);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k] * x[j + k];
+    }
+    for (; j < j1; ++j) acc += __builtin_nontemporal_load(a + (int64_t)(j - j0) * n) * x[j];
+    partial[(int64_t)blockIdx.y * n + i] = acc;
+}
+__global__ __launch_bounds__(256) void k_dense_gemv_sum(const double* __restrict__ partial, int n, int nchunk,
+                                                        double* __restrict__ y)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += partial[(int64_t)c * n + i];
+    y[i] = s;
+}
+}  // namespace
+
+void launch_dense_gemv(const double* A, const double* x, int n, double* partial, double* y, hipStream_t s)
+{
+    const int nchunk = dense_gemv_chunks(n);
+    const unsigned rows = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_dense_gemv_partial, dim3(rows, (unsigned)nchunk), dim3(256), 0, s, A, x, n, partial);
+    TN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_dense_gemv_sum, dim3(rows), dim3(256), 0, s, partial, n, nchunk, y);
     TN_HIP(hipGetLastError());
 }
 template void launch_gather_owned<double>(const double*, int64_t, const int64_t*, int64_t, double*, hipStream_t);

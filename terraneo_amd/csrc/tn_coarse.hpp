@@ -19,6 +19,12 @@ void launch_scatter_owned(T* v, int64_t N, const int64_t* owned_idx, int64_t now
 template <typename T>
 void launch_set_owned_unit(T* v, int64_t N, const int64_t* owned_idx, int c, hipStream_t s);
 
+// y = A x, A dense column-major n x n (n < 65535 * DENSE_GEMV_CHUNK);
+// partial holds dense_gemv_chunks(n) * n doubles of scratch
+constexpr int DENSE_GEMV_CHUNK = 128;
+inline int dense_gemv_chunks(int n) { return (n + DENSE_GEMV_CHUNK - 1) / DENSE_GEMV_CHUNK; }
+void launch_dense_gemv(const double* A, const double* x, int n, double* partial, double* y, hipStream_t s);
+
 class DenseCoarse : public Solver {
   public:
     DenseCoarse(tn_eps* op, hipStream_t s);
@@ -29,7 +35,7 @@ class DenseCoarse : public Solver {
     tn_eps* op_;
     int64_t nown_ = 0, n_ = 0;
     DevBuf<int64_t> owned_idx_;
-    DevBuf<double> inv_, xu_, yu_;
+    DevBuf<double> inv_, xu_, yu_, part_;
     rocblas_handle handle_ = nullptr;
 };
 
