@@ -700,6 +700,7 @@ __global__ __launch_bounds__(256) void k_dense_gemv_sum(const double* __restrict
 
 void launch_dense_gemv(const double* A, const double* x, int n, double* partial, double* y, hipStream_t s)
 {
+    TN_REQUIRE(n > 0 && dense_gemv_chunks(n) <= 65535, "dense gemv: size out of range");
     const int nchunk = dense_gemv_chunks(n);
     const unsigned rows = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_dense_gemv_partial, dim3(rows, (unsigned)nchunk), dim3(256), 0, s, A, x, n, partial);

@@ -209,6 +209,8 @@ def main():
     ap.add_argument("--reps", type=int, default=10)
     a = ap.parse_args()
     if a.what == "slice":
+        if a.trace is None:
+            ap.error("slice needs the kernel trace CSV")
         slice_trace(a.trace, a.reps)
         return
     {"stored": stored, "coupling": coupling, "eps": eps, "solve": solve, "transfers": transfers}[a.what](a.lat, a.reps)
