@@ -287,8 +287,10 @@ __device__ constexpr int WK[2][3] = {{0, 1, 2}, {3, 2, 1}};
 // B: a wave per coarse lateral quad (X, Y) and chunk of 64 coarse radial
 // cells (one per lane).  A lane sums the 16 fine wedges of its coarse cell
 // into the cell's 8 corners; the shared radial level is merged by a lane
-// shuffle, so a lane adds its lower 4 corners to p with one atomic each and
-// the chunk's top lane the upper 4.
+// shuffle, so a lane writes its lower 4 corner sums per cell and the chunk's
+// top lane the upper 4 once per chunk.  k_div_gather then sums, per pressure
+// node copy, the (at most 4) lateral quads' values in a fixed order: no
+// atomics, the same result on every run.
 template <typename T>
 __global__ __launch_bounds__(256) void k_div(DivArgs<T> a)
 {
@@ -366,16 +368,44 @@ __global__ __launch_bounds__(256) void k_div(DivArgs<T> a)
                 }
             }
         }
-    const int nc = a.nc, nrc = a.nrc;
-    const int64_t c00 = ((int64_t)d * nc + X) * nc + Y;
-    const int64_t latc[4] = {c00, c00 + nc, c00 + 1, c00 + nc + 1};
+    const int64_t ncol = 10LL * nqc * nqc, ncell = ncol * ncrc;
+    const int64_t col = ((int64_t)d * nqc + X) * nqc + Y;
+    T* ctop = a.csum + 4 * ncell;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         T below = __shfl_up(acc[k][1], 1, 64);
         if (lane == 0) below = T(0);
-        if (valid) unsafeAtomicAdd(a.p + latc[k] * nrc + R, acc[k][0] + below);
-        if (lane == toplane) unsafeAtomicAdd(a.p + latc[k] * nrc + R + 1, acc[k][1]);
+        if (valid) a.csum[k * ncell + col * ncrc + R] = acc[k][0] + below;
+        if (lane == toplane) ctop[(k * ncol + col) * nch + h] = acc[k][1];
     }
+}
+
+// Pressure node copy (d, X, Y, level): corner k of quad (X - (k & 1), Y - (k >> 1)).
+template <typename T>
+__global__ __launch_bounds__(256) void k_div_gather(DivArgs<T> a)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = a.nc, nrc = a.nrc, nqc = nc - 1, ncrc = nrc - 1;
+    if (tid >= 10LL * nc * nc * nrc) return;
+    const int nch = (ncrc + 63) >> 6;
+    const int64_t ncol = 10LL * nqc * nqc, ncell = ncol * ncrc;
+    const T* ctop = a.csum + 4 * ncell;
+    const int lev = (int)(tid % nrc);
+    const int64_t lat = tid / nrc;
+    const int Yn = (int)(lat % nc);
+    const int Xn = (int)((lat / nc) % nc);
+    const int d = (int)(lat / ((int64_t)nc * nc));
+    T v = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int qx = Xn - (k & 1), qy = Yn - (k >> 1);
+        if (qx < 0 || qy < 0 || qx > nqc - 1 || qy > nqc - 1) continue;
+        const int64_t col = ((int64_t)d * nqc + qx) * nqc + qy;
+        if (lev < ncrc) v += a.csum[k * ncell + col * ncrc + lev];
+        // first level of a chunk > 0, or the top level: the previous chunk's top
+        if ((lev > 0 && (lev & 63) == 0) || lev == ncrc) v += ctop[(k * ncol + col) * nch + ((lev - 1) >> 6)];
+    }
+    a.p[tid] = v;
 }
 
 // B^T: the atomic-free lateral march (tn_march.hpp) over fine cells: each
@@ -547,8 +577,12 @@ void launch_div(const DivArgs<T>& a, hipStream_t s)
 {
     TN_REQUIRE(a.nf == 2 * a.nc - 1 && a.nrf == 2 * a.nrc - 1 && a.nrc >= 2 && a.nc >= 2,
                "B: velocity mesh must be the pressure mesh refined once");
+    TN_REQUIRE(a.csum, "B: corner-sum scratch missing");
     const int64_t waves = 10LL * (a.nc - 1) * (a.nc - 1) * ((a.nrc - 1 + 63) / 64);
     hipLaunchKernelGGL(k_div<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a);
+    TN_HIP(hipGetLastError());
+    const int64_t Nc = 10LL * a.nc * a.nc * a.nrc;
+    hipLaunchKernelGGL(k_div_gather<T>, dim3((unsigned)((Nc + 255) / 256)), dim3(256), 0, s, a);
     TN_HIP(hipGetLastError());
 }
 

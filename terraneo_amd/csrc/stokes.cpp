@@ -31,8 +31,10 @@ void div_t(tn_stokes* K, const void* u, void* p, hipStream_t s)
 {
     auto& mf = const_cast<tn_mesh*>(K->vel)->lvl.dev();
     auto& mc = const_cast<tn_mesh*>(K->pre)->lvl.dev();
-    launch_fill<T>((T*)p, mc.N, T(0), s);
-    DivArgs<T> a{mf.d_coords.p, mf.d_radii.p, mf.n, mf.nr, mc.n, mc.nr, K->A->bc_cmb, K->A->bc_surf, (const T*)u, (T*)p};
+    const size_t bytes = sizeof(T) * (size_t)div_scratch_elems(mc.n, mc.nr);
+    if (K->div_tmp.n != bytes) K->div_tmp.alloc(bytes);
+    DivArgs<T> a{mf.d_coords.p, mf.d_radii.p, mf.n,  mf.nr,         mc.n,         mc.nr,
+                 K->A->bc_cmb,  K->A->bc_surf, (const T*)u, (T*)p, (T*)K->div_tmp.p};
     launch_div<T>(a, s);
     launch_interface_sum<T>(mc, 1, (T*)p, s);
 }

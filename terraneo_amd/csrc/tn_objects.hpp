@@ -37,6 +37,7 @@ struct tn_stokes {
     const tn_mesh* pre = nullptr;
     tn_eps* A = nullptr;  // viscous block on the velocity mesh; its BCs govern B and B^T
     tn::DevBuf<unsigned char> rhs_tmp;  // nodal buoyancy before the mass matrix (tn_stokes_rhs_from_temperature)
+    tn::DevBuf<unsigned char> div_tmp;  // B's per-cell corner sums (div_scratch_elems)
 };
 
 namespace tn {

@@ -12,8 +12,15 @@ struct DivArgs {
     int nf, nrf, nc, nrc;
     int bc_cmb, bc_surf;
     const T* u;  // fine velocity [3][Nf]
-    T* p;        // coarse pressure [Nc] (accumulated)
+    T* p;        // coarse pressure [Nc] (every copy written)
+    T* csum;     // scratch, div_scratch_elems(nc, nrc): per-cell corner sums, then the chunk tops
 };
+// scratch elements of B: 4 corners x (coarse cells + one top level per 64-cell chunk)
+inline int64_t div_scratch_elems(int nc, int nrc)
+{
+    const int64_t ncol = 10LL * (nc - 1) * (nc - 1);
+    return 4 * ncol * ((int64_t)(nrc - 1) + (nrc - 1 + 63) / 64);
+}
 
 template <typename T>
 struct GradArgs {

@@ -50,6 +50,32 @@ def test_div_grad_match_oracle(cuda, bc):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("bc", [(1, 1), (2, 2)])
+def test_div_two_radial_chunks_repeats_bitwise(cuda, bc):
+    """B with 128 coarse radial cells (two 64-cell chunks: the chunk-top sums
+    meet the next chunk's first level) against the oracle, and a second apply
+    equal bit for bit (no atomics in B)."""
+    import torch
+
+    import terraneo_amd as tn
+
+    mv, mp = tn.Mesh(3, 8, 0.55, 1.0), tn.Mesh(2, 7, 0.55, 1.0)
+    ov, op = O.Mesh(3, mv.nr, 0.55, 1.0), O.Mesh(2, mp.nr, 0.55, 1.0)
+    assert mp.nr - 1 == 128
+    eta = smooth_eta(ov.coords, ov.radii).reshape(-1)
+    K = tn.Stokes(mv, mp, tn.Eps(mv, torch.from_numpy(eta).to(cuda), *bc))
+    u = consistent_random(ov, 3, 5).reshape(-1)
+    ud = torch.from_numpy(u).to(cuda)
+    p1 = torch.empty(mp.N, dtype=torch.float64, device=cuda)
+    p2 = torch.full_like(p1, 7.0)
+    K.div(ud, p1)
+    K.div(ud, p2)
+    assert torch.equal(p1, p2)
+    ref = O.div_apply(ov, op, u, *bc)
+    assert np.abs(p1.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
 def test_mass_operators_match_oracle(cuda):
     import torch
 
