@@ -17,6 +17,11 @@
  *
  * Streams: every compute call takes a hipStream_t (NULL = default stream)
  * and is asynchronous with respect to the host unless documented otherwise.
+ * Device-side failures of the matrix-free march kernels (a bounded hand-off
+ * wait between waves that gave up) set a host-visible word; the next
+ * synchronising call (tn_sync, tn_dot, tn_max_abs_entry, tn_has_nan_or_inf,
+ * the solvers) or the next apply returns -1 with that message, and the word
+ * is cleared.
  * Objects are not re-entrant: one stream at a time per object (the
  * reference's operators keep mutable src/dst views, epsilon_divdiv_kerngen.hpp:94-96).
  */
@@ -116,6 +121,10 @@ int tn_free(void* ptr);
 int tn_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int tn_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int tn_sync(void);
+/* Test hook, no counterpart in the reference: polls per bounded hand-off wait
+ * of the march kernels (default 2^20).  A small value forces the waits to give
+ * up, so the error path above can be exercised; every kernel still exits. */
+int tn_set_wait_limit(int polls);
 
 /* ------------------------------------------------------------------ mesh
  * One refinement level of the thick spherical shell, 10 diamonds, one
@@ -189,6 +198,13 @@ int tn_eps_set_eta(tn_eps* op, const void* eta);
 int tn_eps_apply(tn_eps* op, const void* src, void* dst, tn_stream_t s);
 /* penalty epsilon of the fs/fs rotation penalty (set_penalty_epsilon, kerngen.hpp:279) */
 int tn_eps_set_penalty_epsilon(tn_eps* op, double eps);
+/* No counterpart in the reference.  Multiplicity of the fs/fs rotation penalty:
+ * once = 0 (default) adds it at every node copy before the interface sum, as
+ * the reference does (kerngen.hpp:583-613), so an interface node receives it
+ * once per copy and the operator is not symmetric; once = 1 adds it at the
+ * owned copy only (once per node), which makes the operator symmetric, as
+ * CG and MINRES (and a symmetric multigrid preconditioner) require. */
+int tn_eps_set_penalty_once(tn_eps* op, int once);
 /* Galerkin coarse operator from the next finer operator (TwoGridGCA, gca.hpp:72-531;
  * switches `coarse` to its stored-matrix path, set_stored_matrix_mode kerngen.hpp:443-464).
  * Matrices are stored symmetric-packed in TN_F64 or TN_F32. */

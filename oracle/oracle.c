@@ -556,6 +556,13 @@ void orc_eps_destroy(orc_eps* op)
     free(op);
 }
 
+/* Not in the reference: its penalty is added at every node copy before the
+ * interface sum (F5), so an interface node receives it once per copy and the
+ * operator is not symmetric.  once = 1 adds it at the owned copy only. */
+void orc_eps_set_penalty_once(orc_eps* op, int once) { op->penalty_once = once ? 1 : 0; }
+
+void orc_eps_set_penalty_epsilon(orc_eps* op, double eps) { op->penalty_eps = eps; }
+
 void orc_eps_set_stored(orc_eps* op, const double* mats)
 {
     const orc_mesh* m = op->m;
@@ -725,7 +732,8 @@ void orc_eps_apply(const orc_eps* op, const double* x, double* yv, int diagonal,
         for (int i = 0; i < 3; i++) {
             const double* nm = op->null_modes + i * 3 * N;
             const double alpha = op->penalty_eps * orc_dot_owned(m, 3, nm, x);
-            for (int64_t t = 0; t < 3 * N; t++) yv[t] += alpha * nm[t];
+            for (int64_t t = 0; t < 3 * N; t++)
+                if (!op->penalty_once || m->owned[t % N]) yv[t] += alpha * nm[t];
         }
     }
     if (comm) orc_interface_sum(n, nr, 3, m->group_of, m->ngroups, yv);

@@ -64,6 +64,8 @@ typedef struct orc_eps {
     double* stored;       /* optional [10][n-1][n-1][nr-1][2][18*18] (GCA) */
     double* null_modes;   /* fs/fs penalty modes [3][3*N] */
     double penalty_eps;
+    int penalty_once;     /* 0: added at every node copy before the interface sum (reference, F5);
+                             1: added at the owned copy only (once per node: a symmetric operator) */
 } orc_eps;
 
 orc_eps* orc_eps_create(const orc_mesh* m, const double* k, int bc_cmb, int bc_surf);
@@ -78,6 +80,10 @@ void     orc_eps_cell_effective(const orc_eps* op, int d, int x, int y, int r, i
 int64_t  orc_eps_assemble_coo(const orc_eps* op, int64_t* rows, int64_t* cols, double* vals);
 /* gather the fs/fs penalty modes (each 3*N doubles); returns 0 when inactive */
 int      orc_eps_penalty_modes(const orc_eps* op, double* out);
+/* fs/fs penalty multiplicity: once = 0 reference (per copy), 1 once per node (owned copy) */
+void     orc_eps_set_penalty_once(orc_eps* op, int once);
+/* fs/fs penalty epsilon (set_penalty_epsilon, epsilon_divdiv_kerngen.hpp:279) */
+void     orc_eps_set_penalty_epsilon(orc_eps* op, double eps);
 /* per hex cell viscosity (both wedges): kwJ = kc[cell] |det J| */
 void     orc_eps_set_cell_viscosity(orc_eps* op, const double* kc);
 /* adjoint gradient w.r.t. per-cell log-viscosity m = ln kc:

@@ -53,6 +53,7 @@ SIGNATURES = {
     "tn_memcpy_h2d": (i32, [vp, vp, C.c_size_t]),
     "tn_memcpy_d2h": (i32, [vp, vp, C.c_size_t]),
     "tn_sync": (i32, []),
+    "tn_set_wait_limit": (i32, [i32]),
     "tn_mesh_create": (i32, [i32, i32, f64, f64, pvp]),
     "tn_mesh_create_radii": (i32, [i32, i32, C.POINTER(f64), pvp]),
     "tn_mesh_destroy": (i32, [vp]),
@@ -78,6 +79,7 @@ SIGNATURES = {
     "tn_eps_set_eta": (i32, [vp, vp]),
     "tn_eps_apply": (i32, [vp, vp, vp, vp]),
     "tn_eps_set_penalty_epsilon": (i32, [vp, f64]),
+    "tn_eps_set_penalty_once": (i32, [vp, i32]),
     "tn_eps_gca_from": (i32, [vp, vp, i32, vp]),
     "tn_eps_get_local_matrix": (i32, [vp, i32, i32, i32, i32, i32, vp]),
     "tn_eps_set_cell_eta": (i32, [vp, vp]),

@@ -195,6 +195,11 @@ class Eps:
     def set_penalty_epsilon(self, eps):
         check(lib().tn_eps_set_penalty_epsilon(self.h, eps))
 
+    def set_penalty_once(self, once=True):
+        """fs/fs rotation penalty at the owned copy only (symmetric operator) instead of
+        at every node copy (the reference's rule)"""
+        check(lib().tn_eps_set_penalty_once(self.h, int(bool(once))))
+
 
 # ---------------------------------------------------------------- mesh-level helpers
 def vector_mass(mesh, x, y, stream=None):

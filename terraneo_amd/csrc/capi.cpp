@@ -1,6 +1,8 @@
 // C ABI: mesh, level-1 operations and the epsilon-div-div operator.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <mutex>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -14,8 +16,35 @@
 namespace tn {
 namespace {
 thread_local std::string g_last_error;
-}
+// host-visible word of the march kernels' bounded waits (pinned, coherent)
+int* g_wait_fail_host = nullptr;
+int* g_wait_fail_dev = nullptr;
+std::atomic<int> g_wait_limit{1 << 20};
+std::mutex g_wait_mutex;
+}  // namespace
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+MarchGuard march_guard()
+{
+    std::lock_guard<std::mutex> lk(g_wait_mutex);
+    if (!g_wait_fail_host) {
+        int* h = nullptr;
+        TN_HIP(hipHostMalloc((void**)&h, sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+        __atomic_store_n(h, 0, __ATOMIC_RELEASE);
+        void* d = nullptr;
+        TN_HIP(hipHostGetDevicePointer(&d, h, 0));
+        g_wait_fail_host = h;
+        g_wait_fail_dev = (int*)d;
+    }
+    return {g_wait_fail_dev, g_wait_limit.load()};
+}
+
+void check_march_status()
+{
+    int* h = g_wait_fail_host;
+    if (h && __atomic_exchange_n(h, 0, __ATOMIC_ACQ_REL))
+        throw Error("march kernel: a hand-off wait between waves gave up (output of an earlier apply is invalid)");
+}
 }  // namespace tn
 
 #define TN_API_BEGIN try {
@@ -158,9 +187,12 @@ void eps_apply_t(tn_eps* op, const void* src_v, void* dst_v, hipStream_t s)
             op->blk_ready = true;
         }
         const tn::Geom<T> g = geom_of<T>(m);
-        tn::MarchArgs<T> a{src, dst, eta, eta_cell, g.geo, g.coords, g.radii, m.d_coords.p, m.d_radii.p,
+        // penalty positions: every copy (the reference's rule) or the owned copy only
+        const double* pen_coords = op->penalty_once ? m.d_coords_owned.p : m.d_coords.p;
+        tn::MarchArgs<T> a{src, dst, eta, eta_cell, g.geo, g.coords, g.radii, pen_coords, m.d_radii.p,
                            need_blk ? (const T*)op->blk.p : nullptr, penalty ? op->pen.p : nullptr,
-                           m.n, m.nr, march_seg(m.n, (int)sizeof(T)), op->bc_cmb, op->bc_surf, add ? 1 : 0};
+                           m.n, m.nr, march_seg(m.n, (int)sizeof(T)), op->bc_cmb, op->bc_surf, add ? 1 : 0,
+                           tn::march_guard()};
         tn::launch_eps_march<T>(a, op->diagonal, s);
     } else {
         // the gather writes every node copy once (boundary test side and penalty fused)
@@ -178,7 +210,8 @@ void eps_apply_t(tn_eps* op, const void* src_v, void* dst_v, hipStream_t s)
             using S = std::remove_const_t<std::remove_pointer_t<decltype(stored)>>;
             tn::StoredArgs<T, S> a{stored, src, dst, m.d_coords.p, m.n, m.nr, op->bc_cmb, op->bc_surf, add ? 1 : 0,
                                    csum, ctop, need_blk ? (const T*)op->blk.p : nullptr,
-                                   penalty ? op->pen.p : nullptr, m.d_radii.p};
+                                   penalty ? op->pen.p : nullptr, m.d_radii.p,
+                                   op->penalty_once ? m.d_coords_owned.p : m.d_coords.p};
             if (need_blk && !op->blk_ready) {
                 tn::launch_stored_blk<T, S>(a, (T*)op->blk.p, s);
                 op->blk_ready = true;
@@ -239,6 +272,15 @@ int tn_sync(void)
 {
     TN_API_BEGIN
     TN_HIP(hipDeviceSynchronize());
+    tn::check_march_status();
+    TN_API_END
+}
+
+int tn_set_wait_limit(int polls)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(polls >= 0, "wait limit must be >= 0");
+    tn::g_wait_limit.store(polls == 0 ? 0 : polls);
     TN_API_END
 }
 
@@ -363,11 +405,14 @@ int tn_dot(const tn_mesh* m, int dtype, int ncomp, const void* a, const void* b,
     auto& L = const_cast<tn_mesh*>(m)->lvl.dev();
     tn::launch_fill<double>(L.d_scratch.p, 1, 0.0, (hipStream_t)s);
     if (dtype == TN_F64)
-        tn::launch_dot_owned<double>(L, ncomp, (const double*)a, (const double*)b, L.d_scratch.p, (hipStream_t)s);
+        tn::launch_dot_owned<double>(L, ncomp, (const double*)a, (const double*)b, L.d_scratch.p, L.d_part.p,
+                                     (hipStream_t)s);
     else
-        tn::launch_dot_owned<float>(L, ncomp, (const float*)a, (const float*)b, L.d_scratch.p, (hipStream_t)s);
+        tn::launch_dot_owned<float>(L, ncomp, (const float*)a, (const float*)b, L.d_scratch.p, L.d_part.p,
+                                    (hipStream_t)s);
     TN_HIP(hipMemcpyAsync(out, L.d_scratch.p, sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)s));
     TN_HIP(hipStreamSynchronize((hipStream_t)s));
+    tn::check_march_status();
     TN_API_END
 }
 
@@ -420,6 +465,7 @@ int tn_max_abs_entry(const tn_mesh* m, int dtype, int ncomp, const void* v, doub
         tn::launch_max_abs<float>((const float*)v, cnt, L.d_scratch.p, (hipStream_t)s);
     TN_HIP(hipMemcpyAsync(out, L.d_scratch.p, sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)s));
     TN_HIP(hipStreamSynchronize((hipStream_t)s));
+    tn::check_march_status();
     TN_API_END
 }
 
@@ -437,6 +483,7 @@ int tn_has_nan_or_inf(const tn_mesh* m, int dtype, int ncomp, const void* v, int
         tn::launch_nonfinite<float>((const float*)v, cnt, flag, (hipStream_t)s);
     TN_HIP(hipMemcpyAsync(out, flag, sizeof(int), hipMemcpyDeviceToHost, (hipStream_t)s));
     TN_HIP(hipStreamSynchronize((hipStream_t)s));
+    tn::check_march_status();
     TN_API_END
 }
 
@@ -534,7 +581,16 @@ int tn_eps_apply(tn_eps* op, const void* src, void* dst, tn_stream_t s)
 {
     TN_API_BEGIN
     TN_REQUIRE(op, "null operator");
+    tn::check_march_status();
     tn::eps_apply(op, src, dst, (hipStream_t)s);
+    TN_API_END
+}
+
+int tn_eps_set_penalty_once(tn_eps* op, int once)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op, "null operator");
+    op->penalty_once = once ? 1 : 0;
     TN_API_END
 }
 

@@ -4,6 +4,7 @@
 #include <memory>
 
 #include "tn_coarse.hpp"
+#include "tn_eps_march.hpp"
 #include "tn_mg.hpp"
 #include "tn_objects.hpp"
 #include "tn_solvers.hpp"
@@ -62,9 +63,9 @@ struct MeanFree {
     int dtype;
     VecStore st;
     BVec ones;
-    DevBuf<double> sum;
+    DevBuf<double> sum, part;
     double inv_count = 0.0;
-    MeanFree(Ctx& c, const Field& like) : m(like.m), dtype(like.dtype), sum(1)
+    MeanFree(Ctx& c, const Field& like) : m(like.m), dtype(like.dtype), sum(1), part(REDUCE_PART_DOUBLES)
     {
         ones = st.make(vec1(like));
         v_fill(c, ones, 1.0);
@@ -74,10 +75,10 @@ struct MeanFree {
     {
         launch_fill<double>(sum.p, 1, 0.0, c.s);
         if (dtype == TN_F64) {
-            launch_dot_owned<double>(*m, 1, (const double*)p, (const double*)ones.b[0].p, sum.p, c.s);
+            launch_dot_owned<double>(*m, 1, (const double*)p, (const double*)ones.b[0].p, sum.p, part.p, c.s);
             launch_sub_scaled<double>((double*)p, m->N, sum.p, inv_count, c.s);
         } else {
-            launch_dot_owned<float>(*m, 1, (const float*)p, (const float*)ones.b[0].p, sum.p, c.s);
+            launch_dot_owned<float>(*m, 1, (const float*)p, (const float*)ones.b[0].p, sum.p, part.p, c.s);
             launch_sub_scaled<float>((float*)p, m->N, sum.p, inv_count, c.s);
         }
     }
@@ -338,6 +339,7 @@ int tn_pcg_solve(tn_eps* A, tn_gmg* prec, void* x, const void* b, const tn_krylo
     pcg.solve(c, op, X, B);
     TN_HIP(hipStreamSynchronize(c.s));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    tn::check_march_status();
     fill_info(info, pcg.info, secs, history, history_cap);
     TN_API_END
 }
@@ -438,6 +440,7 @@ int tn_stokes_solve(tn_stokes* K, int method, tn_gmg* prec_u, const void* schur_
     if (mf) mf->project(c, X.b[1].p);
     TN_HIP(hipStreamSynchronize(c.s));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    tn::check_march_status();
     fill_info(info, it, secs, history, history_cap);
     if (info) {
         info->restarts = restarts;

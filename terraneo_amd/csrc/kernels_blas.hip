@@ -48,19 +48,42 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// Block-reduce `v` into out[slot] (double atomics into device memory).
-__device__ inline void block_accumulate(double v, double* out)
+// Block total of `v`, valid in thread 0 (no atomics).
+__device__ inline double block_total(double v)
 {
     __shared__ double part[16];
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) part[wv] = v;
     __syncthreads();
+    double s = 0.0;
     if (wv == 0) {
         const int nw = blockDim.x >> 6;
-        double s = lane < nw ? part[lane] : 0.0;
+        s = lane < nw ? part[lane] : 0.0;
         s = wave_sum(s);
-        if (lane == 0) unsafeAtomicAdd(out, s);
+    }
+    __syncthreads();
+    return s;
+}
+
+// The block's total of `v` into part[blockIdx.x].  Reductions leave one
+// partial per block and k_reduce_finalize adds them in a fixed order, so a
+// reduction gives the same bits on every run (no floating-point atomics).
+__device__ inline void block_partial(double v, double* __restrict__ part)
+{
+    const double t = block_total(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// out[k] += sum over b < nblk of part[k * nblk + b], k < nk (one block)
+__global__ __launch_bounds__(256) void k_reduce_finalize(const double* __restrict__ part, int nblk, int nk,
+                                                         double* __restrict__ out)
+{
+    for (int k = 0; k < nk; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(int64_t)k * nblk + b];
+        const double t = block_total(s);
+        if (threadIdx.x == 0) out[k] += t;
     }
 }
 
@@ -80,28 +103,10 @@ struct ColumnWalk {
     }
 };
 
-// Block total of `v`, valid in thread 0 (no atomics).
-__device__ inline double block_total(double v)
-{
-    __shared__ double part[16];
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) part[wv] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (wv == 0) {
-        const int nw = blockDim.x >> 6;
-        s = lane < nw ? part[lane] : 0.0;
-        s = wave_sum(s);
-    }
-    __syncthreads();
-    return s;
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_dot_owned(const T* __restrict__ a, const T* __restrict__ b,
                                                    const uint8_t* __restrict__ owned_lat, int64_t N, int nr,
-                                                   int ncomp, double* out)
+                                                   int ncomp, double* __restrict__ part)
 {
     double s = 0.0;
     const int lane = threadIdx.x & 63;
@@ -111,13 +116,13 @@ __global__ __launch_bounds__(256) void k_dot_owned(const T* __restrict__ a, cons
         const T* bc = b + cw.col * nr;
         for (int r = lane; r < nr; r += 64) s += (double)ac[r] * (double)bc[r];
     }
-    block_accumulate(s, out);
+    block_partial(s, part);
 }
 
-// out += sum over the owned entries of a one-component field (column walk)
+// sum over the owned entries of a one-component field (column walk), per block
 template <typename T>
 __global__ __launch_bounds__(256) void k_sum_owned(const T* __restrict__ a, const uint8_t* __restrict__ owned_lat,
-                                                   int64_t N, int nr, double* out)
+                                                   int64_t N, int nr, double* __restrict__ part)
 {
     double s = 0.0;
     const int lane = threadIdx.x & 63;
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(256) void k_sum_owned(const T* __restrict__ a, cons
         const T* ac = a + cw.col * nr;
         for (int r = lane; r < nr; r += 64) s += (double)ac[r];
     }
-    block_accumulate(s, out);
+    block_partial(s, part);
 }
 
 // One Chebyshev step fused (chebyshev.hpp:100-168 does it in four passes):
@@ -151,12 +156,12 @@ __global__ __launch_bounds__(256) void k_cheby_update(T* __restrict__ x, T* __re
     }
 }
 
-// w += alpha x, then out += sum_owned w . y (y == w: the new w's squared norm):
+// w += alpha x, then the per-block sums of sum_owned w . y (y == w: the new w's squared norm):
 // one modified Gram-Schmidt step fused with the next step's dot.
 template <typename T, bool SELF>
 __global__ __launch_bounds__(256) void k_axpy_dot(T* __restrict__ w, double alpha, const T* __restrict__ x,
                                                   const T* __restrict__ y, const uint8_t* __restrict__ owned_lat,
-                                                  int64_t N, int nr, int ncomp, double* out)
+                                                  int64_t N, int nr, int ncomp, double* __restrict__ part)
 {
     double s = 0.0;
     const T a = (T)alpha;
@@ -172,7 +177,7 @@ __global__ __launch_bounds__(256) void k_axpy_dot(T* __restrict__ w, double alph
             if (own) s += (double)wn * (double)(SELF ? wn : yc[r]);
         }
     }
-    block_accumulate(s, out);
+    block_partial(s, part);
 }
 
 // y[i] -= scale * sum[0] (sum: a device scalar from a preceding reduction)
@@ -375,11 +380,12 @@ __global__ __launch_bounds__(256) void k_max_abs(const T* __restrict__ y, int64_
     if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
-// out[k] += sum_owned a . b_k for k < nb (multi-dot, one pass over a)
+// per-block sums of sum_owned a . b_k into part[k * gridDim.x + block], k < nb
+// (multi-dot, one pass over a)
 template <typename T, int NB>
 __global__ __launch_bounds__(256) void k_multidot(const T* __restrict__ a, MultiDotArgs<T> bs,
                                                   const uint8_t* __restrict__ owned_lat, int64_t N, int nr, int ncomp,
-                                                  double* out)
+                                                  double* __restrict__ part)
 {
     double s[NB];
 #pragma unroll
@@ -398,8 +404,7 @@ __global__ __launch_bounds__(256) void k_multidot(const T* __restrict__ a, Multi
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         if (k >= bs.nb) break;
-        block_accumulate(s[k], out + k);
-        __syncthreads();
+        block_partial(s[k], part + (int64_t)k * gridDim.x);
     }
 }
 
@@ -442,20 +447,31 @@ void launch_lincomb(int64_t count, T* y, int nx, const double* c, const T* const
     TN_HIP(hipGetLastError());
 }
 
-template <typename T>
-void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, double* out_dev, hipStream_t s)
+static void launch_reduce_finalize(const double* part, int nblk, int nk, double* out_dev, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dot_owned<T>, dim3(grid_for(m.N / m.nr * ncomp, 4, 2048)), dim3(256), 0, s, a, b,
-                       m.d_owned_lat.p, m.N, m.nr, ncomp, out_dev);
+    hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(256), 0, s, part, nblk, nk, out_dev);
     TN_HIP(hipGetLastError());
 }
 
+int reduce_blocks(const MeshLevel& m, int ncomp) { return (int)grid_for(m.N / m.nr * ncomp, 4, REDUCE_MAX_BLOCKS); }
+
 template <typename T>
-void launch_sum_owned(const MeshLevel& m, const T* a, double* out_dev, hipStream_t s)
+void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, double* out_dev, double* part_dev,
+                      hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sum_owned<T>, dim3(grid_for(m.N / m.nr, 4, 2048)), dim3(256), 0, s, a, m.d_owned_lat.p, m.N,
-                       m.nr, out_dev);
+    const int g = reduce_blocks(m, ncomp);
+    hipLaunchKernelGGL(k_dot_owned<T>, dim3(g), dim3(256), 0, s, a, b, m.d_owned_lat.p, m.N, m.nr, ncomp, part_dev);
     TN_HIP(hipGetLastError());
+    launch_reduce_finalize(part_dev, g, 1, out_dev, s);
+}
+
+template <typename T>
+void launch_sum_owned(const MeshLevel& m, const T* a, double* out_dev, double* part_dev, hipStream_t s)
+{
+    const int g = reduce_blocks(m, 1);
+    hipLaunchKernelGGL(k_sum_owned<T>, dim3(g), dim3(256), 0, s, a, m.d_owned_lat.p, m.N, m.nr, part_dev);
+    TN_HIP(hipGetLastError());
+    launch_reduce_finalize(part_dev, g, 1, out_dev, s);
 }
 
 template <typename T>
@@ -474,16 +490,18 @@ void launch_cheby_update(T* x, T* d, const T* az, const T* b, const T* dinv, int
 
 template <typename T>
 void launch_axpy_dot(const MeshLevel& m, int ncomp, T* w, double alpha, const T* x, const T* y, double* out_dev,
-                     hipStream_t s)
+                     double* part_dev, hipStream_t s)
 {
-    const dim3 gr(grid_for(m.N / m.nr * ncomp, 4, 2048)), bl(256);
+    const int g = reduce_blocks(m, ncomp);
+    const dim3 gr(g), bl(256);
     if (y == w)
         hipLaunchKernelGGL((k_axpy_dot<T, true>), gr, bl, 0, s, w, alpha, x, y, m.d_owned_lat.p, m.N, m.nr, ncomp,
-                           out_dev);
+                           part_dev);
     else
         hipLaunchKernelGGL((k_axpy_dot<T, false>), gr, bl, 0, s, w, alpha, x, y, m.d_owned_lat.p, m.N, m.nr, ncomp,
-                           out_dev);
+                           part_dev);
     TN_HIP(hipGetLastError());
+    launch_reduce_finalize(part_dev, g, 1, out_dev, s);
 }
 
 template <typename T>
@@ -569,17 +587,20 @@ void launch_max_abs(const T* y, int64_t count, double* out_dev, hipStream_t s)
 
 template <typename T>
 void launch_multidot(const MeshLevel& m, int ncomp, const T* a, const MultiDotArgs<T>& bs, double* out_dev,
-                     hipStream_t s)
+                     double* part_dev, hipStream_t s)
 {
     TN_REQUIRE(bs.nb >= 1 && bs.nb <= MultiDotArgs<T>::MAX, "multidot: 1..16 vectors");
-    const dim3 gr(grid_for(m.N / m.nr * ncomp, 4, 1024)), bl(256);
+    const int g = (int)grid_for(m.N / m.nr * ncomp, 4, REDUCE_MAX_BLOCKS / 2);
+    static_assert(MultiDotArgs<T>::MAX * (REDUCE_MAX_BLOCKS / 2) <= REDUCE_PART_DOUBLES, "multidot partials");
+    const dim3 gr(g), bl(256);
     if (bs.nb <= 4)
-        hipLaunchKernelGGL((k_multidot<T, 4>), gr, bl, 0, s, a, bs, m.d_owned_lat.p, m.N, m.nr, ncomp, out_dev);
+        hipLaunchKernelGGL((k_multidot<T, 4>), gr, bl, 0, s, a, bs, m.d_owned_lat.p, m.N, m.nr, ncomp, part_dev);
     else if (bs.nb <= 8)
-        hipLaunchKernelGGL((k_multidot<T, 8>), gr, bl, 0, s, a, bs, m.d_owned_lat.p, m.N, m.nr, ncomp, out_dev);
+        hipLaunchKernelGGL((k_multidot<T, 8>), gr, bl, 0, s, a, bs, m.d_owned_lat.p, m.N, m.nr, ncomp, part_dev);
     else
-        hipLaunchKernelGGL((k_multidot<T, 16>), gr, bl, 0, s, a, bs, m.d_owned_lat.p, m.N, m.nr, ncomp, out_dev);
+        hipLaunchKernelGGL((k_multidot<T, 16>), gr, bl, 0, s, a, bs, m.d_owned_lat.p, m.N, m.nr, ncomp, part_dev);
     TN_HIP(hipGetLastError());
+    launch_reduce_finalize(part_dev, g, bs.nb, out_dev, s);
 }
 
 template void launch_fill<int>(int*, int64_t, int, hipStream_t);
@@ -589,10 +610,11 @@ template void launch_cast<float, double>(float*, const double*, int64_t, hipStre
 #define TN_INST(T)                                                                                               \
     template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                                   \
     template void launch_lincomb<T>(int64_t, T*, int, const double*, const T* const*, double, hipStream_t);     \
-    template void launch_dot_owned<T>(const MeshLevel&, int, const T*, const T*, double*, hipStream_t);          \
+    template void launch_dot_owned<T>(const MeshLevel&, int, const T*, const T*, double*, double*, hipStream_t); \
     template void launch_sub_scaled<T>(T*, int64_t, const double*, double, hipStream_t);                         \
-    template void launch_sum_owned<T>(const MeshLevel&, const T*, double*, hipStream_t);                          \
-    template void launch_axpy_dot<T>(const MeshLevel&, int, T*, double, const T*, const T*, double*, hipStream_t); \
+    template void launch_sum_owned<T>(const MeshLevel&, const T*, double*, double*, hipStream_t);                 \
+    template void launch_axpy_dot<T>(const MeshLevel&, int, T*, double, const T*, const T*, double*, double*,    \
+                                     hipStream_t);                                                               \
     template void launch_cheby_update<T>(T*, T*, const T*, const T*, const T*, int64_t, double, double, int,    \
                                          hipStream_t);                                                           \
     template void launch_interface_sum<T>(const MeshLevel&, int, T*, hipStream_t);                               \
@@ -603,7 +625,8 @@ template void launch_cast<float, double>(float*, const double*, int64_t, hipStre
     template void launch_invert<T>(T*, int64_t, hipStream_t);                                                    \
     template void launch_randomize<T>(T*, int64_t, uint64_t, hipStream_t);                                       \
     template void launch_nonfinite<T>(const T*, int64_t, int*, hipStream_t);                                     \
-    template void launch_multidot<T>(const MeshLevel&, int, const T*, const MultiDotArgs<T>&, double*, hipStream_t);
+    template void launch_multidot<T>(const MeshLevel&, int, const T*, const MultiDotArgs<T>&, double*, double*, \
+                                     hipStream_t);
 TN_INST(double)
 TN_INST(float)
 #undef TN_INST

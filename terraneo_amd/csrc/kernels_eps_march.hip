@@ -241,7 +241,7 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
             }
             // radial merge (node level c = bottom of cell c + top of cell c-1) and routing;
             // the slot is free once the next wave has read what it held RING steps ago
-            if (w + 1 < W) march::wait_counter(&used[w + 1], step - RING + 1);
+            if (w + 1 < W) march::wait_counter(&used[w + 1], step - RING + 1, a.guard);
             T part[3], tp[3] = {0, 0, 0};
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                 }
             }
             march::post_counter(&done[w], step + 1, lane);
-            if (w >= 1) march::wait_counter(&done[w - 1], step + 1);
+            if (w >= 1) march::wait_counter(&done[w - 1], step + 1, a.guard);
             if (row_own && qy >= y0) {
                 const T(*Ein)[65] = ebuf[step % RING][w - 1];
                 const int64_t lat = lat00;
@@ -311,6 +311,413 @@ __global__ __launch_bounds__(64 * W) void k_eps_march(MarchArgs<T> a, const T* _
                     const V3<T> v{tp[0] + Ein[0][64], tp[1] + Ein[1][64], tp[2] + Ein[2][64]};
                     if (last) {
                         finish(v, ncr, tmode, blk1, prad_top);
+                    } else {
+                        lvl[w][slot][0] = v.x;
+                        lvl[w][slot][1] = v.y;
+                        lvl[w][slot][2] = v.z;
+                    }
+                }
+            }
+            march::post_counter(&used[w], step + 1, lane);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Matrix apply (not the diagonal mode) with the corner data carried along the
+// march.  The quads (x, y) and (x, y+1) of a strip share the corners
+// (x, y+1), (x+1, y+1), so a step loads only its two new corners and keeps
+// their per-corner sums S = x_bot + x_top, d = gamma (x_top - x_bot),
+// e = eta_bot + eta_top for the next step (half the loads of the form above).
+// Fields are read and written through buffer resources: a radial column is
+// addressed by a 32-bit wave-uniform byte offset (scalar register) plus the
+// lane's byte offset, so no load or store carries per-lane address
+// arithmetic, and the two node levels of a lane are one 16-byte load (FP64).
+// Accesses past a resource's size return zero / are dropped by the hardware.
+// The radial scalings are folded out of the wedge:
+//   with a_i = l_i / r_mid, zz = z / (dr/2), G = U1 (x) a1 + U2 (x) a2 + D (x) zz
+//   = G' / r_mid with G' = U1 (x) l1 + U2 (x) l2 + gamma D (x) z, gamma = r_mid / (dr/2),
+//   and s = K (2 sym G' - 2/3 tr G' I), K = (sum of the six nodal eta) (dr/2)/6 |det A|
+//   (= kwJ / r_mid^2), the outputs are p_i = s l_i and q = gamma s z, so the
+//   bottom / top node values p_i -/+ q are single fused multiply-adds.
+// FP64 VALU operations per hex cell: 20 (two new corners) + 22 (wedge inputs)
+// + 2 x 75 (wedges) + 42 (outputs) + 36 (radial merge) + 9 (routing) = 279.
+template <typename T>
+struct CornerSum {
+    T S[3], d[3], e;
+};
+
+template <typename T>
+struct CornerRawCol {
+    T xb[3], xt[3], eb, et;
+};
+
+// 64-bit buffer access through a resource, byte offsets (lane: voff, uniform: soff)
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, double)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+// buffer resource over `bytes` bytes at p (raw, unswizzled; gfx9 word 3)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+
+// one lane's two node levels of a radial column: cb = column byte offset
+// (lat * nr * sizeof(T)), Nb = component stride in bytes, voff = lane's level byte offset
+template <typename T>
+__device__ __forceinline__ void load_corner(CornerRawCol<T>& r, __amdgpu_buffer_rsrc_t src,
+                                            __amdgpu_buffer_rsrc_t eta, uint32_t cb, uint32_t Nb, uint32_t voff,
+                                            bool nodal)
+{
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        r.xb[j] = buf_ld(src, voff, cb + j * Nb, T(0));
+        r.xt[j] = buf_ld(src, voff + (uint32_t)sizeof(T), cb + j * Nb, T(0));
+    }
+    if (nodal) {
+        r.eb = buf_ld(eta, voff, cb, T(0));
+        r.et = buf_ld(eta, voff + (uint32_t)sizeof(T), cb, T(0));
+    } else {
+        r.eb = r.et = T(0);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void make_corner(CornerSum<T>& c, const CornerRawCol<T>& r, T gam)
+{
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        c.S[j] = r.xb[j] + r.xt[j];
+        c.d[j] = gam * (r.xt[j] - r.xb[j]);
+    }
+    c.e = r.eb + r.et;
+}
+
+// one wedge: s = K (2 sym G' - 2/3 tr G' I) with G' = U1 (x) l1 + U2 (x) l2 + D (x) z
+// (D carries gamma); p1 = s l1, p2 = s l2, q = s z (q without gamma)
+template <typename T>
+__device__ __forceinline__ void wedge_folded(const T U1[3], const T U2[3], const T D[3], T K, const T* __restrict__ g,
+                                             T p1[3], T p2[3], T q[3])
+{
+    T G[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) G[k][l] = U1[k] * g[l] + U2[k] * g[3 + l] + D[k] * g[6 + l];
+    const T tr = G[0][0] + G[1][1] + G[2][2];
+    const T K2 = K + K;
+    const T kt = (T(2.0 / 3.0) * K) * tr;
+    const T s00 = K2 * G[0][0] - kt, s11 = K2 * G[1][1] - kt, s22 = K2 * G[2][2] - kt;
+    const T s01 = K * (G[0][1] + G[1][0]), s02 = K * (G[0][2] + G[2][0]), s12 = K * (G[1][2] + G[2][1]);
+    p1[0] = s00 * g[0] + s01 * g[1] + s02 * g[2];
+    p1[1] = s01 * g[0] + s11 * g[1] + s12 * g[2];
+    p1[2] = s02 * g[0] + s12 * g[1] + s22 * g[2];
+    p2[0] = s00 * g[3] + s01 * g[4] + s02 * g[5];
+    p2[1] = s01 * g[3] + s11 * g[4] + s12 * g[5];
+    p2[2] = s02 * g[3] + s12 * g[4] + s22 * g[5];
+    q[0] = s00 * g[6] + s01 * g[7] + s02 * g[8];
+    q[1] = s01 * g[6] + s11 * g[7] + s12 * g[8];
+    q[2] = s02 * g[6] + s12 * g[7] + s22 * g[8];
+}
+
+// Neumann cell from its four corner sums (0 (x,y), 1 (x+1,y), 2 (x,y+1),
+// 3 (x+1,y+1); wedges (0,1,2) and (3,2,1), kernel_helpers.hpp:302-327).
+// hd6 = (dr/2)/6; ecell6 = 6 eta_cell, or a negative value for the nodal viscosity.
+template <typename T>
+__device__ __forceinline__ void cell_folded(const CornerSum<T>& c0, const CornerSum<T>& c1, const CornerSum<T>& c2,
+                                            const CornerSum<T>& c3, const T* __restrict__ geo, T hd6, T gam,
+                                            T ecell6, T yb[4][3], T yt[4][3])
+{
+    T d12[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d12[k] = c1.d[k] + c2.d[k];
+    const T e12 = c1.e + c2.e;
+    T p1a[3], p2a[3], qa[3], p1b[3], p2b[3], qb[3];
+    {
+        const T* g = geo;
+        T U1[3], U2[3], D[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            U1[k] = c1.S[k] - c0.S[k];
+            U2[k] = c2.S[k] - c0.S[k];
+            D[k] = c0.d[k] + d12[k];
+        }
+        const T es = ecell6 >= T(0) ? ecell6 : c0.e + e12;
+        wedge_folded(U1, U2, D, es * (hd6 * g[9]), g, p1a, p2a, qa);
+    }
+    {
+        const T* g = geo + GEO_STRIDE;
+        T U1[3], U2[3], D[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            U1[k] = c2.S[k] - c3.S[k];
+            U2[k] = c1.S[k] - c3.S[k];
+            D[k] = c3.d[k] + d12[k];
+        }
+        const T es = ecell6 >= T(0) ? ecell6 : c3.e + e12;
+        wedge_folded(U1, U2, D, es * (hd6 * g[9]), g, p1b, p2b, qb);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const T t0 = p1a[k] + p2a[k];
+        yb[0][k] = -t0 - gam * qa[k];
+        yt[0][k] = gam * qa[k] - t0;
+        const T t3 = p1b[k] + p2b[k];
+        yb[3][k] = -t3 - gam * qb[k];
+        yt[3][k] = gam * qb[k] - t3;
+        const T Q = qa[k] + qb[k];
+        const T P1 = p1a[k] + p2b[k];
+        yb[1][k] = P1 - gam * Q;
+        yt[1][k] = P1 + gam * Q;
+        const T P2 = p2a[k] + p1b[k];
+        yb[2][k] = P2 - gam * Q;
+        yt[2][k] = P2 + gam * Q;
+    }
+}
+
+// lane i receives lane i-1's value, lane 0 receives 0 (DPP wavefront shift right by one)
+__device__ __forceinline__ float shift_up1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double shift_up1(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <typename T, int W, bool PF>
+__global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __restrict__ geo_tab,
+                                                     const T* __restrict__ coords, const T* __restrict__ blk_tab,
+                                                     const double* __restrict__ coords_d,
+                                                     const double* __restrict__ pen)
+{
+    constexpr int OWN = W - 1;
+    constexpr int RING = (sizeof(T) == 8 && W > 12) ? 3 : 4;
+    constexpr uint32_t SZ = sizeof(T);
+    __shared__ T ebuf[RING][W][3][65];       // corner (x+1, y) hand-off; slot 64: chunk-top level
+    __shared__ int done[W], used[W];
+    __shared__ T lvl[W][MARCH_SEG_MAX][3];   // chunk-top level values awaiting the next chunk
+    __shared__ T carry[W][2][3][64];         // carried output sums (x, y+1), (x+1, y+1)
+    __shared__ T carry_top[W][2][3];
+    __shared__ T xkeep[W][2][2][3];          // raw boundary-level source of node (x, y): [parity][cmb / surface]
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int n = a.n, nr = a.nr, ncr = nr - 1, ncl = n - 1;
+    const int nxb = (n + OWN - 1) / OWN, nys = (n + a.seg - 1) / a.seg;
+    int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int ys = b % nys;
+    b /= nys;
+    const int xb = b % nxb;
+    const int d = b / nxb;
+    const int x = xb * OWN - 1 + w;
+    const bool strip_ok = x >= 0 && x <= n - 2;
+    const bool row_own = w >= 1 && x <= n - 1;
+    const int y0 = ys * a.seg, y1 = min(y0 + a.seg, n);
+    const uint32_t Nn = 10u * n * n * nr;    // nodes per component
+    const uint32_t Nb = Nn * SZ;
+    const __amdgpu_buffer_rsrc_t rs_src = buf_rsrc(a.src, 3 * Nb);
+    const __amdgpu_buffer_rsrc_t rs_dst = buf_rsrc(a.dst, 3 * Nb);
+    const bool nodal = a.eta_cell == nullptr;
+    const __amdgpu_buffer_rsrc_t rs_eta = buf_rsrc(nodal ? (const void*)a.eta : (const void*)a.src, nodal ? Nb : 0u);
+    const int bmode = a.bc_cmb, tmode = a.bc_surf;
+    const int nchunks = (ncr + 63) >> 6;
+    // byte offsets of radial columns: node (xc, yc) of this diamond at ((d n + xc) n + yc) nr
+    const uint32_t colstep = (uint32_t)nr * SZ;             // next y
+    const uint32_t rowstep = (uint32_t)n * nr * SZ;         // next x
+    const uint32_t colx = (uint32_t)((d * n + x) * n) * nr * SZ;  // column (x, 0) (x >= -1: wraps for x = -1, unused)
+    if (lane == 0) done[w] = used[w] = 0;
+    __syncthreads();
+    int step = 0;
+
+    for (int h = 0; h < nchunks; ++h) {
+        const int c = 64 * h + lane;
+        const bool valid = c < ncr;
+        const int toplane = min(63, ncr - 1 - 64 * h);
+        const bool last = h == nchunks - 1;
+        const int cc = valid ? c : ncr - 1;
+        const uint32_t voff = (uint32_t)cc * SZ;
+        const T r0 = a.radii[cc], r1 = a.radii[cc + 1];
+        const T half_dr = T(0.5) * (r1 - r0), r_mid = T(0.5) * (r0 + r1);
+        const T gam = r_mid / half_dr, hd6 = half_dr * T(1.0 / 6.0);
+        // boundary lanes of this chunk (trial side on load, test side at the store)
+        const bool bot_b = h == 0 && bmode != TN_NEUMANN && lane == 0;
+        const bool top_b = last && tmode != TN_NEUMANN && c == ncr - 1;
+        const bool any_b = (h == 0 && bmode != TN_NEUMANN) || (last && tmode != TN_NEUMANN);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            carry[w][0][j][lane] = T(0);
+            carry[w][1][j][lane] = T(0);
+            if (lane == 0) carry_top[w][0][j] = carry_top[w][1][j] = T(0);
+        }
+        // corner data: raw columns -> sums (trial rule of the boundary levels applied)
+        auto process = [&](CornerSum<T>& cs, CornerRawCol<T>& r, int xc, int yc, bool keep) {
+            if (any_b) {
+                const int64_t lat = ((int64_t)d * n + xc) * n + yc;
+                if (keep) {
+                    if (bot_b) {
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) xkeep[w][yc & 1][0][j] = r.xb[j];
+                    }
+                    if (top_b) {
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) xkeep[w][yc & 1][1][j] = r.xt[j];
+                    }
+                }
+                if (bot_b) apply_trial(r.xb, bmode, coords, lat);
+                if (top_b) apply_trial(r.xt, tmode, coords, lat);
+            }
+            make_corner(cs, r, gam);
+        };
+        CornerSum<T> C0, C1, N0, N1;
+        CornerRawCol<T> R0, R1;
+        {
+            const int yp = y0 - 1;
+            if (strip_ok && yp >= 0) {
+                const uint32_t cb = colx + (uint32_t)yp * colstep;
+                load_corner(R0, rs_src, rs_eta, cb, Nb, voff, nodal);
+                load_corner(R1, rs_src, rs_eta, cb + rowstep, Nb, voff, nodal);
+                process(C0, R0, x, yp, true);
+                process(C1, R1, x + 1, yp, false);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) C0.S[j] = C0.d[j] = C1.S[j] = C1.d[j] = T(0);
+                C0.e = C1.e = T(0);
+            }
+            if (PF && strip_ok && y0 <= n - 1) {
+                const uint32_t cb = colx + (uint32_t)y0 * colstep;
+                load_corner(R0, rs_src, rs_eta, cb, Nb, voff, nodal);
+                load_corner(R1, rs_src, rs_eta, cb + rowstep, Nb, voff, nodal);
+            }
+        }
+        int goff = ((d * ncl + x) * ncl + (y0 - 1)) * 2 * GEO_STRIDE;
+
+        for (int qy = y0 - 1; qy <= y1 - 1; ++qy, ++step, goff += 2 * GEO_STRIDE) {
+            const bool quad_ok = strip_ok && qy >= 0 && qy <= n - 2;
+            const bool next_ok = strip_ok && qy + 1 <= n - 1;
+            T(*E)[65] = ebuf[step % RING][w];
+            if (next_ok) {
+                if (!PF) {
+                    const uint32_t cb = colx + (uint32_t)(qy + 1) * colstep;
+                    load_corner(R0, rs_src, rs_eta, cb, Nb, voff, nodal);
+                    load_corner(R1, rs_src, rs_eta, cb + rowstep, Nb, voff, nodal);
+                }
+                process(N0, R0, x, qy + 1, true);
+                process(N1, R1, x + 1, qy + 1, false);
+                if (PF && qy + 2 <= n - 1 && qy + 1 <= y1 - 1) {
+                    const uint32_t cb = colx + (uint32_t)(qy + 2) * colstep;
+                    load_corner(R0, rs_src, rs_eta, cb, Nb, voff, nodal);
+                    load_corner(R1, rs_src, rs_eta, cb + rowstep, Nb, voff, nodal);
+                }
+            }
+            T yb[4][3], yt[4][3];
+            if (quad_ok) {
+                const T* gsrc = geo_tab + __builtin_amdgcn_readfirstlane(goff);
+                T geo[2 * GEO_STRIDE];
+#pragma unroll
+                for (int i = 0; i < 2 * GEO_STRIDE; ++i) geo[i] = gsrc[i];
+                const T ecell6 =
+                    nodal ? T(-1) : T(6) * a.eta_cell[(((int64_t)d * ncl + x) * ncl + qy) * ncr + cc];
+                cell_folded(C0, C1, N0, N1, geo, hd6, gam, ecell6, yb, yt);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
+            }
+            if (next_ok) {
+                C0 = N0;
+                C1 = N1;
+            }
+            if (w + 1 < W) march::wait_counter(&used[w + 1], step - RING + 1, a.guard);
+            T part[3], tp[3] = {0, 0, 0};
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                T M[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) M[k] = yb[k][j] + shift_up1(yt[k][j]);
+                part[j] = M[0] + carry[w][0][j][lane];
+                carry[w][0][j][lane] = M[2];
+                E[j][lane] = M[1] + carry[w][1][j][lane];
+                carry[w][1][j][lane] = M[3];
+            }
+            if (lane == toplane) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    tp[j] = yt[0][j] + carry_top[w][0][j];
+                    carry_top[w][0][j] = yt[2][j];
+                    E[j][64] = yt[1][j] + carry_top[w][1][j];
+                    carry_top[w][1][j] = yt[3][j];
+                }
+            }
+            march::post_counter(&done[w], step + 1, lane);
+            if (w >= 1) march::wait_counter(&done[w - 1], step + 1, a.guard);
+            if (row_own && qy >= y0) {
+                const T(*Ein)[65] = ebuf[step % RING][w - 1];
+                const uint32_t cb = colx + (uint32_t)qy * colstep;
+                const int slot = qy - y0;
+                auto finish = [&](V3<T> v, int level, int mode, int side) {
+                    if (mode != TN_NEUMANN || pen) {
+                        const int64_t lat = ((int64_t)d * n + x) * n + qy;
+                        if (mode != TN_NEUMANN) {
+                            const T* xk = xkeep[w][qy & 1][side];
+                            const V3<T> xv{xk[0], xk[1], xk[2]};
+                            const T* blk = blk_tab + ((int64_t)side * 10 * n * n + lat) * 3;
+                            v = test_value<T, false>(v, xv, mode, coords, lat, blk);
+                        }
+                        if (pen) {
+                            // fs/fs rotation penalty y += gamma x p, gamma = S beta (FP64)
+                            const double* bt = pen;
+                            const double* S = pen + 4;
+                            const double g0 = S[0] * bt[0] + S[1] * bt[1] + S[2] * bt[2];
+                            const double g1 = S[3] * bt[0] + S[4] * bt[1] + S[5] * bt[2];
+                            const double g2 = S[6] * bt[0] + S[7] * bt[1] + S[8] * bt[2];
+                            const double r = a.radii_d[level];
+                            const double px = coords_d[3 * lat] * r, py = coords_d[3 * lat + 1] * r,
+                                         pz = coords_d[3 * lat + 2] * r;
+                            v.x += (T)(g1 * pz - g2 * py);
+                            v.y += (T)(g2 * px - g0 * pz);
+                            v.z += (T)(g0 * py - g1 * px);
+                        }
+                    }
+                    const uint32_t lo = (uint32_t)level * SZ;
+                    if (a.add) {
+                        v.x += buf_ld(rs_dst, lo, cb, T(0));
+                        v.y += buf_ld(rs_dst, lo, cb + Nb, T(0));
+                        v.z += buf_ld(rs_dst, lo, cb + 2 * Nb, T(0));
+                    }
+                    buf_st(v.x, rs_dst, lo, cb);
+                    buf_st(v.y, rs_dst, lo, cb + Nb);
+                    buf_st(v.z, rs_dst, lo, cb + 2 * Nb);
+                };
+                if (valid) {
+                    V3<T> v{part[0] + Ein[0][lane], part[1] + Ein[1][lane], part[2] + Ein[2][lane]};
+                    if (lane == 0 && h > 0) {
+                        v.x += lvl[w][slot][0];
+                        v.y += lvl[w][slot][1];
+                        v.z += lvl[w][slot][2];
+                    }
+                    finish(v, c, c == 0 ? bmode : TN_NEUMANN, 0);
+                }
+                if (lane == toplane) {
+                    const V3<T> v{tp[0] + Ein[0][64], tp[1] + Ein[1][64], tp[2] + Ein[2][64]};
+                    if (last) {
+                        finish(v, ncr, tmode, 1);
                     } else {
                         lvl[w][slot][0] = v.x;
                         lvl[w][slot][1] = v.y;
@@ -441,14 +848,32 @@ int march_block_rows(int dtype_size) { return (dtype_size == 8 ? march_waves<dou
 
 int march_waves_per_cu(int dtype_size) { return march_block_rows(dtype_size) + 1; }
 
+// matrix-apply kernel variant: 0 the per-quad-load march, 1 the carried-corner
+// march, 2 the same with the next step's loads issued one step ahead
+// (TN_EPS_KERNEL overrides; tuning)
+int eps_kernel_variant()
+{
+    static const int v = [] {
+        const char* e = std::getenv("TN_EPS_KERNEL");
+        const int k = e ? std::atoi(e) : 0;
+        return (k >= 0 && k <= 2) ? k : 0;
+    }();
+    return v;
+}
+
 template <typename T, int W>
 void launch_march_w(const MarchArgs<T>& a, bool diagonal, hipStream_t s)
 {
     constexpr int OWN = W - 1;
     const int nxb = (a.n + OWN - 1) / OWN, nys = (a.n + a.seg - 1) / a.seg;
     const dim3 gr((unsigned)(10 * nxb * nys)), bl(64 * W);
+    const int var = eps_kernel_variant();
     if (diagonal)
         hipLaunchKernelGGL((k_eps_march<T, true, W>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen);
+    else if (var == 1)
+        hipLaunchKernelGGL((k_eps_flow<T, W, false>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen);
+    else if (var == 2)
+        hipLaunchKernelGGL((k_eps_flow<T, W, true>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen);
     else
         hipLaunchKernelGGL((k_eps_march<T, false, W>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen);
     TN_HIP(hipGetLastError());

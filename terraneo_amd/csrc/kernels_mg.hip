@@ -761,7 +761,8 @@ __global__ __launch_bounds__(256) void k_stored_gather(StoredArgs<T, S> a)
         const double g1 = Sm[3] * bt[0] + Sm[4] * bt[1] + Sm[5] * bt[2];
         const double g2 = Sm[6] * bt[0] + Sm[7] * bt[1] + Sm[8] * bt[2];
         const double r = a.radii[lev];
-        const double px = a.coords[3 * lat] * r, py = a.coords[3 * lat + 1] * r, pz = a.coords[3 * lat + 2] * r;
+        const double px = a.pen_coords[3 * lat] * r, py = a.pen_coords[3 * lat + 1] * r,
+                     pz = a.pen_coords[3 * lat + 2] * r;
         v[0] += (T)(g1 * pz - g2 * py);
         v[1] += (T)(g2 * px - g0 * pz);
         v[2] += (T)(g0 * py - g1 * px);

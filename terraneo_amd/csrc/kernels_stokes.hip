@@ -504,7 +504,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3))) voi
                                                          const T* __restrict__ p, T* __restrict__ u)
 {
     GradCell<T> op(a, coords, p, u);
-    march::run<T, W>(op, a.nf, a.nrf, a.seg);
+    march::run<T, W>(op, a.nf, a.nrf, a.seg, a.guard);
 }
 
 // VectorMass (ncomp components) / lumped k-weighted mass (LUMPED_K).

@@ -237,6 +237,7 @@ void MeshLevel::upload()
     d_grp_off.upload(grp_off);
     d_grp_mem.upload(grp_mem);
     d_scratch.alloc(64);
+    d_part.alloc(REDUCE_PART_DOUBLES);
     uploaded = true;
 }
 

@@ -11,6 +11,7 @@
 //   block preconditioners        linalg/solvers/block_preconditioner_2x2.hpp:91-95, 205-221
 //   DiagonalSolver               linalg/solvers/diagonal_solver.hpp:34-49
 #include "tn_solvers.hpp"
+#include "tn_eps_march.hpp"
 #include "tn_mg.hpp"
 
 #include <algorithm>
@@ -44,6 +45,7 @@ VecStore::~VecStore()
 Ctx::Ctx()
 {
     red.alloc(32);
+    part.alloc(REDUCE_PART_DOUBLES);
     flag.alloc(1);
 }
 
@@ -52,6 +54,7 @@ double Ctx::fetch(int slots, double* out)
     double h[32];
     TN_HIP(hipMemcpyAsync(h, red.p, sizeof(double) * slots, hipMemcpyDeviceToHost, s));
     TN_HIP(hipStreamSynchronize(s));
+    check_march_status();
     if (out)
         for (int i = 0; i < slots; ++i) out[i] = h[i];
     return h[0];
@@ -99,9 +102,10 @@ double v_dot(Ctx& c, const BVec& a, const BVec& b)
     for (int i = 0; i < a.nb; ++i) {
         const Field& f = a.b[i];
         if (f.dtype == TN_F64)
-            launch_dot_owned<double>(*f.m, f.ncomp, (const double*)f.p, (const double*)b.b[i].p, c.red.p, c.s);
+            launch_dot_owned<double>(*f.m, f.ncomp, (const double*)f.p, (const double*)b.b[i].p, c.red.p, c.part.p,
+                                     c.s);
         else
-            launch_dot_owned<float>(*f.m, f.ncomp, (const float*)f.p, (const float*)b.b[i].p, c.red.p, c.s);
+            launch_dot_owned<float>(*f.m, f.ncomp, (const float*)f.p, (const float*)b.b[i].p, c.red.p, c.part.p, c.s);
     }
     return c.fetch(1);
 }
@@ -113,10 +117,10 @@ double v_axpy_dot(Ctx& c, const BVec& w, double alpha, const BVec& x, const BVec
         const Field& f = w.b[i];
         if (f.dtype == TN_F64)
             launch_axpy_dot<double>(*f.m, f.ncomp, (double*)f.p, alpha, (const double*)x.b[i].p,
-                                    (const double*)y.b[i].p, c.red.p, c.s);
+                                    (const double*)y.b[i].p, c.red.p, c.part.p, c.s);
         else
             launch_axpy_dot<float>(*f.m, f.ncomp, (float*)f.p, alpha, (const float*)x.b[i].p, (const float*)y.b[i].p,
-                                   c.red.p, c.s);
+                                   c.red.p, c.part.p, c.s);
     }
     return c.fetch(1);
 }
@@ -132,12 +136,12 @@ void v_multidot(Ctx& c, const BVec& a, const std::vector<const BVec*>& bs, doubl
             MultiDotArgs<double> args{};
             args.nb = n;
             for (int k = 0; k < n; ++k) args.b[k] = (const double*)bs[k]->b[i].p;
-            launch_multidot<double>(*f.m, f.ncomp, (const double*)f.p, args, c.red.p, c.s);
+            launch_multidot<double>(*f.m, f.ncomp, (const double*)f.p, args, c.red.p, c.part.p, c.s);
         } else {
             MultiDotArgs<float> args{};
             args.nb = n;
             for (int k = 0; k < n; ++k) args.b[k] = (const float*)bs[k]->b[i].p;
-            launch_multidot<float>(*f.m, f.ncomp, (const float*)f.p, args, c.red.p, c.s);
+            launch_multidot<float>(*f.m, f.ncomp, (const float*)f.p, args, c.red.p, c.part.p, c.s);
         }
     }
     c.fetch(n, out);
@@ -189,6 +193,7 @@ bool v_nonfinite(Ctx& c, const BVec& y)
     int h = 0;
     TN_HIP(hipMemcpyAsync(&h, c.flag.p, sizeof(int), hipMemcpyDeviceToHost, c.s));
     TN_HIP(hipStreamSynchronize(c.s));
+    check_march_status();
     return h != 0;
 }
 

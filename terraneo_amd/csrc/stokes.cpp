@@ -31,8 +31,7 @@ void div_t(tn_stokes* K, const void* u, void* p, hipStream_t s)
 {
     auto& mf = const_cast<tn_mesh*>(K->vel)->lvl.dev();
     auto& mc = const_cast<tn_mesh*>(K->pre)->lvl.dev();
-    const size_t bytes = sizeof(T) * (size_t)div_scratch_elems(mc.n, mc.nr);
-    if (K->div_tmp.n != bytes) K->div_tmp.alloc(bytes);
+    TN_REQUIRE(K->div_tmp.n == sizeof(T) * (size_t)div_scratch_elems(mc.n, mc.nr), "B: scratch not allocated");
     DivArgs<T> a{mf.d_coords.p, mf.d_radii.p, mf.n,  mf.nr,         mc.n,         mc.nr,
                  K->A->bc_cmb,  K->A->bc_surf, (const T*)u, (T*)p, (T*)K->div_tmp.p};
     launch_div<T>(a, s);
@@ -47,7 +46,7 @@ void grad_t(tn_stokes* K, const void* p, void* u, bool add, bool comm, hipStream
     constexpr int W = grad_march_waves<T>();
     const int seg = march_seg_for(mf.n, W - 1, sizeof(T) == 8 ? W : 2 * W);
     GradArgs<T> a{mf.d_coords.p, mf.d_radii.p, mf.n,          mf.nr,        mc.n, mc.nr, K->A->bc_cmb,
-                  K->A->bc_surf, (const T*)p,   (T*)u, seg, add ? 1 : 0};
+                  K->A->bc_surf, (const T*)p,   (T*)u, seg, add ? 1 : 0, march_guard()};
     launch_grad<T>(a, s);
     if (comm) launch_interface_sum<T>(mf, 3, (T*)u, s);
 }
@@ -161,6 +160,9 @@ int tn_stokes_create(const tn_mesh* vel, const tn_mesh* pre, tn_eps* A, tn_stoke
     K->vel = vel;
     K->pre = pre;
     K->A = A;
+    // B's scratch (per-cell corner sums) is allocated here, not in the apply path
+    const size_t esize = A->dtype == TN_F64 ? sizeof(double) : sizeof(float);
+    K->div_tmp.alloc(esize * (size_t)tn::div_scratch_elems(pre->lvl.n, pre->lvl.nr));
     *out = K.release();
     TN_API_END
 }
@@ -176,6 +178,7 @@ int tn_stokes_apply(tn_stokes* K, const void* u, const void* p, void* yu, void* 
 {
     TN_API_BEGIN
     TN_REQUIRE(K && u && p && yu && yp, "null pointer");
+    tn::check_march_status();
     tn::stokes_apply(K, u, p, yu, yp, (hipStream_t)s);
     TN_API_END
 }
@@ -195,6 +198,7 @@ int tn_grad_apply(tn_stokes* K, const void* p, void* u, int apply_mode, int comm
 {
     TN_API_BEGIN
     TN_REQUIRE(K && u && p, "null pointer");
+    tn::check_march_status();
     const bool add = apply_mode == TN_ADD, comm = comm_mode == TN_COMM_ADDITIVE;
     if (K->A->dtype == TN_F64)
         tn::grad_t<double>(K, p, u, add, comm, (hipStream_t)s);
@@ -377,10 +381,10 @@ int tn_pressure_normalize(const tn_mesh* m, int dtype, void* p, tn_stream_t s)
     tn::launch_fill<double>(sum, 1, 0.0, st);
     const double inv_owned = 1.0 / (double)L.owned;
     if (dtype == TN_F64) {
-        tn::launch_sum_owned<double>(L, (const double*)p, sum, st);
+        tn::launch_sum_owned<double>(L, (const double*)p, sum, L.d_part.p, st);
         tn::launch_sub_scaled<double>((double*)p, L.N, sum, inv_owned, st);
     } else {
-        tn::launch_sum_owned<float>(L, (const float*)p, sum, st);
+        tn::launch_sum_owned<float>(L, (const float*)p, sum, L.d_part.p, st);
         tn::launch_sub_scaled<float>((float*)p, L.N, sum, inv_owned, st);
     }
     TN_API_END

@@ -20,6 +20,19 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks)
     return xcd * per + min(xcd, rem) + local;
 }
 
+// Bounded hand-off waits of the march kernels.  A wait that polls `limit`
+// times without seeing its neighbour's post gives up; the wave then stores 1
+// to *fail, a host-visible word, and finishes normally (its output is not to
+// be trusted).  The host reports the word through check_march_status().
+struct MarchGuard {
+    int* fail;  // device address of the host-visible word
+    int limit;  // polls per wait
+};
+// host: the process's guard (the word is allocated on first use)
+MarchGuard march_guard();
+// host: throws tn::Error if a march wait gave up since the last check (and clears the word)
+void check_march_status();
+
 template <typename T>
 struct MarchArgs {
     const T* src;             // [3][N]
@@ -29,7 +42,7 @@ struct MarchArgs {
     const T* geo;             // [10][n-1][n-1][2][GEO_STRIDE]
     const T* coords;          // [10][n][n][3] unit lateral coordinates
     const T* radii;           // [nr]
-    const double* coords_d;   // FP64 copies for the rotation penalty
+    const double* coords_d;   // FP64 lateral positions where the rotation penalty is added (0: not added)
     const double* radii_d;
     const T* blk;             // [2][10 n n][3] boundary quadratic forms, or null (both shells Neumann)
     const double* pen;        // fs/fs penalty: [0..3) beta, [4..13) S, or null
@@ -37,6 +50,7 @@ struct MarchArgs {
     int seg;                  // node columns per block in the march direction
     int bc_cmb, bc_surf;
     int add;                  // 1: dst += A x, 0: dst = A x
+    MarchGuard guard;
 };
 
 // node rows owned per block (waves per block - 1) for an element size in bytes

@@ -91,7 +91,8 @@ struct MeshLevel {
     DevBuf<uint8_t> d_owned_lat;
     DevBuf<int32_t> d_grp_off;
     DevBuf<int32_t> d_grp_mem;
-    DevBuf<double> d_scratch;  // reduction scratch (doubles)
+    DevBuf<double> d_scratch;  // reduction results (doubles)
+    DevBuf<double> d_part;     // per-block partial sums of the reductions (REDUCE_PART_DOUBLES)
     bool uploaded = false;
 
     int ngroups() const { return (int)grp_off.size() - 1; }
@@ -128,9 +129,19 @@ void launch_interface_sum(const MeshLevel& m, int ncomp, T* v, hipStream_t s);
 template <typename T>
 void launch_lincomb(int64_t count, T* y, int nx, const double* c, const T* const* x, double c0, hipStream_t s);
 
-// out_dev[0..nd) += sum over owned nodes, accumulate in double; ncomp fields of N each
+// Reductions over owned nodes accumulate in double and are deterministic: the
+// kernel leaves one partial sum per block in part_dev (REDUCE_PART_DOUBLES
+// doubles of device scratch) and a one-block pass adds them to out_dev in a
+// fixed order.  Calls that share part_dev must be ordered (one stream).
+constexpr int REDUCE_MAX_BLOCKS = 2048;
+constexpr int REDUCE_PART_DOUBLES = 16 * (REDUCE_MAX_BLOCKS / 2);
+// blocks (= partial sums) of a dot over ncomp fields of mesh m
+int reduce_blocks(const MeshLevel& m, int ncomp);
+
+// out_dev[0] += sum over owned nodes of a . b; ncomp fields of N each
 template <typename T>
-void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, double* out_dev, hipStream_t s);
+void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, double* out_dev, double* part_dev,
+                      hipStream_t s);
 // tangential projection of a vector field's nodes on shell `which` (1 CMB, 2 surface)
 template <typename T>
 void launch_freeslip_project(const MeshLevel& m, T* v, int which, hipStream_t s);
@@ -138,7 +149,7 @@ void launch_freeslip_project(const MeshLevel& m, T* v, int which, hipStream_t s)
 // w += alpha x; *out_dev += sum_owned w . y (y == w: squared norm of the new w)
 template <typename T>
 void launch_axpy_dot(const MeshLevel& m, int ncomp, T* w, double alpha, const T* x, const T* y, double* out_dev,
-                     hipStream_t s);
+                     double* part_dev, hipStream_t s);
 // Chebyshev step: z = dinv (b - az); d = alpha z + beta d; x += d.
 // mode 0: general; 1: first step (d = alpha z); 2: first step from x = 0
 // (az and x not read, x = d written; az may be null)
@@ -151,7 +162,7 @@ template <typename T>
 void launch_sub_scaled(T* y, int64_t count, const double* sum_dev, double scale, hipStream_t s);
 // out_dev[0] += sum of a one-component field over owned nodes
 template <typename T>
-void launch_sum_owned(const MeshLevel& m, const T* a, double* out_dev, hipStream_t s);
+void launch_sum_owned(const MeshLevel& m, const T* a, double* out_dev, double* part_dev, hipStream_t s);
 
 // fs/fs penalty: beta[0..3) = sum_owned r x u  ;  y (+)= gamma x r with gamma = S beta
 template <typename T>
@@ -182,9 +193,10 @@ template <typename T>
 void launch_max_abs(const T* y, int64_t count, double* out_dev, hipStream_t s);
 template <typename T>
 void launch_nonfinite(const T* y, int64_t count, int* flag_dev, hipStream_t s);
+// out_dev[k] += sum_owned a . bs.b[k], k < bs.nb
 template <typename T>
 void launch_multidot(const MeshLevel& m, int ncomp, const T* a, const MultiDotArgs<T>& bs, double* out_dev,
-                     hipStream_t s);
+                     double* part_dev, hipStream_t s);
 
 }  // namespace tn
 

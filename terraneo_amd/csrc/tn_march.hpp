@@ -47,14 +47,18 @@ __host__ __device__ inline int blocks_for(int n, int seg, int waves)
 // instead of a block barrier per step: the waves of a SIMD drift apart and hide
 // each other's memory latency.  The waves of a workgroup are co-resident and
 // every wave posts both counters at every step, so each wait is satisfied
-// after at most RING - 1 steps of a neighbour; the bound (about 40 ms at
-// 2.4 GHz) only guarantees that a wait ends whatever happens.
-__device__ __forceinline__ void wait_counter(int* c, int target)
+// after at most RING - 1 steps of a neighbour; the bound (g.limit polls, by
+// default 2^20, about 40 ms at 2.4 GHz) only guarantees that a wait ends
+// whatever happens.  A wait that gives up flags it in the host-visible word
+// g.fail (a vector store; the host turns it into an error).
+__device__ __forceinline__ void wait_counter(int* c, int target, const MarchGuard& g)
 {
-    for (int i = 0; i < (1 << 20); ++i) {
+    for (int i = 0; i < g.limit; ++i) {
         if (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return;
         __builtin_amdgcn_s_sleep(1);
     }
+    if (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return;
+    if (g.fail) __hip_atomic_store(g.fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __device__ __forceinline__ void post_counter(int* c, int value, int lane)
@@ -63,7 +67,7 @@ __device__ __forceinline__ void post_counter(int* c, int value, int lane)
 }
 
 template <typename T, int W, class Op>
-__device__ __forceinline__ void run(Op& op, int n, int nr, int seg)
+__device__ __forceinline__ void run(Op& op, int n, int nr, int seg, const MarchGuard& guard)
 {
     constexpr int OWN = W - 1;
     constexpr int RING = 4;                  // hand-off slots: a producer may lead by RING - 1 steps
@@ -106,7 +110,7 @@ __device__ __forceinline__ void run(Op& op, int n, int nr, int seg)
             if (quad_ok) op.cell(d, x, qy, yb, yt);
             // radial merge (node level c = bottom of cell c + top of cell c-1) and routing;
             // the slot is free once the next wave has read what it held RING steps ago
-            if (w + 1 < W) wait_counter(&used[w + 1], step - RING + 1);
+            if (w + 1 < W) wait_counter(&used[w + 1], step - RING + 1, guard);
             T part[3], tp[3] = {0, 0, 0};
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
@@ -132,7 +136,7 @@ __device__ __forceinline__ void run(Op& op, int n, int nr, int seg)
                 }
             }
             post_counter(&done[w], step + 1, lane);
-            if (w >= 1) wait_counter(&done[w - 1], step + 1);
+            if (w >= 1) wait_counter(&done[w - 1], step + 1, guard);
             if (row_own && qy >= y0) {
                 const T(*Ein)[65] = ebuf[step % RING][w - 1];
                 const int64_t lat = ((int64_t)d * n + x) * n + qy;

@@ -50,6 +50,7 @@ struct StoredArgs {
     const T* blk;        // [2][10 n n][3] boundary-node quadratic forms (launch_stored_blk), or null
     const double* pen;   // fs/fs penalty: [0..3) beta, [4..13) S, or null
     const double* radii; // [nr] (penalty positions)
+    const double* pen_coords;  // [10][n][n][3] lateral positions where the penalty is added (0: not added)
 };
 
 // boundary-node quadratic forms e^T B e of the stored operator (B: the 3x3

@@ -12,6 +12,9 @@ struct tn_eps {
     int apply_mode = TN_REPLACE, comm_mode = TN_COMM_ADDITIVE;
     bool diagonal = false;
     double penalty_eps = 1.0;
+    // 0: the penalty is added at every node copy before the interface sum (the
+    // reference's rule); 1: at the owned copy only (once per node, symmetric)
+    int penalty_once = 0;
     // fs/fs rotation penalty: y += eps * (G^{-1} beta) x p, beta = sum_owned p x u
     tn::DevBuf<double> pen;  // [0..3) beta, [4..13) S = eps G^{-1}
     bool penalty_ready = false;

@@ -48,7 +48,8 @@ class VecStore {
 
 struct Ctx {
     hipStream_t s = nullptr;
-    DevBuf<double> red;  // reduction slots
+    DevBuf<double> red;   // reduction slots
+    DevBuf<double> part;  // per-block partial sums of the reductions
     DevBuf<int> flag;
     Ctx();
     double fetch(int slots, double* out = nullptr);

@@ -1,6 +1,7 @@
 // Argument blocks of the Stokes coupling / mass kernels.
 #pragma once
 
+#include "tn_eps_march.hpp"
 #include "tn_internal.hpp"
 
 namespace tn {
@@ -32,6 +33,7 @@ struct GradArgs {
     T* u;        // fine velocity
     int seg;     // node columns per block of the march
     int add;     // 1: u += B^T p, 0: u = B^T p
+    MarchGuard guard;
 };
 
 // waves per block of the B^T march (node rows per block + 1): FP64 12 (one

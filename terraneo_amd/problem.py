@@ -32,7 +32,7 @@ class StokesProblem:
 
     def __init__(self, lat, rad, levels, r_min=0.55, r_max=1.0, bc=(TN_FREESLIP, TN_FREESLIP), device="cuda:0",
                  prec_dtype="f64", coarsest_f64=True, gmg=None, gca_bracket="reference", coarse_ops="gca",
-                 mg_penalty=None):
+                 mg_penalty=None, penalty="reference"):
         torch = _torch()
         assert levels >= 2 and lat - levels + 1 >= 0 and rad - levels + 1 >= 1
         self.device = torch.device(device)
@@ -55,6 +55,11 @@ class StokesProblem:
         # epsilon = 1 they leave it as the mesh is refined).  The outer Stokes
         # operator always keeps epsilon = 1.
         self.mg_penalty = mg_penalty
+        # fs/fs rotation penalty multiplicity: "reference" (every node copy, the
+        # reference's rule: the operator is then not symmetric) or "once" (the
+        # owned copy only: a symmetric operator, as MINRES and CG require)
+        assert penalty in ("reference", "once")
+        self.penalty = penalty
         self.gmg_opts = gmg or api.gmg_options(cheby_order=3, cheby_iterations=1, power_iterations=10, num_cycles=1)
         self.f64 = dict(dtype=torch.float64, device=self.device)
         self.A = None
@@ -108,6 +113,9 @@ class StokesProblem:
                 op.gca_from(ops[-1], sdt, bracket=self.gca_bracket)
             ops.append(op)
         self.ops = list(reversed(ops))
+        if self.penalty == "once":
+            for op in self.ops + [self.A]:
+                op.set_penalty_once(True)
         if self.mg_penalty is not None and bc == (TN_FREESLIP, TN_FREESLIP):
             for op, m in zip(self.ops, self.meshes):
                 op.set_penalty_epsilon(float(self.mg_penalty) * self._mean_diagonal(op, m))

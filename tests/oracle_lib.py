@@ -47,6 +47,8 @@ def lib():
         L.orc_eps_set_stored.argtypes = [vp, dp]
         L.orc_eps_local_matrix_k.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp]
         L.orc_eps_set_cell_viscosity.argtypes = [vp, dp]
+        L.orc_eps_set_penalty_once.argtypes = [vp, C.c_int]
+        L.orc_eps_set_penalty_epsilon.argtypes = [vp, C.c_double]
         L.orc_eps_grad_log_cell_viscosity.argtypes = [vp, dp, dp, dp]
         L.orc_eps_penalty_modes.argtypes = [vp, dp]
         L.orc_eps_penalty_modes.restype = C.c_int
@@ -157,6 +159,13 @@ class Eps:
         A = np.zeros(324)
         lib().orc_eps_local_matrix(self.m.h, self.k, d, x, y, r, w, A)
         return A.reshape(18, 18)
+
+    def set_penalty_once(self, once=True):
+        """fs/fs penalty at the owned copy only (symmetric operator) instead of every copy"""
+        lib().orc_eps_set_penalty_once(self.h, int(bool(once)))
+
+    def set_penalty_epsilon(self, eps):
+        lib().orc_eps_set_penalty_epsilon(self.h, float(eps))
 
     def set_stored(self, mats):
         lib().orc_eps_set_stored(self.h, np.ascontiguousarray(mats, dtype=np.float64).reshape(-1))
