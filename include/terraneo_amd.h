@@ -159,6 +159,17 @@ int tn_lincomb(const tn_mesh* m, int dtype, int ncomp, void* y, int nx, const do
                const void* const* x, double c0, tn_stream_t s);
 /* result written to host *out (synchronises the stream) */
 int tn_dot(const tn_mesh* m, int dtype, int ncomp, const void* a, const void* b, double* out, tn_stream_t s);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * tn_dot with its per-block partial sums and result slot in a static per-GPU
+ * device array (16 640 bytes: at most 2048 partial sums, which the launch
+ * never exceeds) instead of the mesh's allocated buffers; calls no hipMalloc.
+ * The mesh must already be resident on the device (any earlier device call on
+ * it uploads it; this entry point does not).  ncomp 1..3.  Calls that overlap
+ * on one GPU (two streams or threads) race on the shared array.  Result on the
+ * host as tn_dot; bit-identical to tn_dot. */
+int tn_dot_noalloc(const tn_mesh* m, int dtype, int ncomp, const void* a, const void* b, double* out,
+                   tn_stream_t s);
 /* v = 1 / v entrywise (invert_entries, grid_operations.hpp:252-270) */
 int tn_invert_entries(const tn_mesh* m, int dtype, int ncomp, void* v, tn_stream_t s);
 /* y = y * d entrywise (scale_with_vector / mult, grid_operations.hpp:272-294) */

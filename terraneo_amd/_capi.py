@@ -66,6 +66,7 @@ SIGNATURES = {
     "tn_interface_sum": (i32, [vp, i32, i32, vp, vp]),
     "tn_lincomb": (i32, [vp, i32, i32, vp, i32, C.POINTER(f64), pvp, f64, vp]),
     "tn_dot": (i32, [vp, i32, i32, vp, vp, C.POINTER(f64), vp]),
+    "tn_dot_noalloc": (i32, [vp, i32, i32, vp, vp, C.POINTER(f64), vp]),
     "tn_invert_entries": (i32, [vp, i32, i32, vp, vp]),
     "tn_scale_with_vector": (i32, [vp, i32, i32, vp, vp, vp]),
     "tn_randomize": (i32, [vp, i32, i32, vp, C.c_uint64, vp]),
