@@ -136,6 +136,14 @@ int tn_set_wait_limit(int polls);
  * Coordinates are bit-identical to the reference generator.
  */
 int tn_mesh_create(int lat_level, int rad_level, double r_min, double r_max, tn_mesh** out);
+/* DistributedDomain::create_uniform(lat, rad, r_min, r_max, s_lat, s_rad)
+ * (spherical_shell.hpp:2547-2588): diamonds split into 4^s_lat lateral and
+ * 2^s_rad radial subdomains.  Only s_lat = s_rad = 0 (one subdomain per
+ * diamond, the layout of every entry point here) is accepted; other splits
+ * return an error (a caller remaps split-subdomain fields to the per-diamond
+ * layout, which holds the same nodes). */
+int tn_mesh_create_subdomains(int lat_level, int rad_level, double r_min, double r_max, int s_lat, int s_rad,
+                              tn_mesh** out);
 /* explicit radii (host array of nr values, increasing) */
 int tn_mesh_create_radii(int lat_level, int nr, const double* radii, tn_mesh** out);
 int tn_mesh_destroy(tn_mesh* m);
@@ -232,9 +240,21 @@ int tn_gca_collect_elements(const tn_mesh* fine, const tn_mesh* coarsest, int dt
  * matrices of `coarse` (its geometry and nodal viscosity, as the reference's on-the-fly assembly for cells
  * without a stored matrix); flagged cells keep their Galerkin matrices.  flags in `coarse`'s dtype. */
 int tn_eps_gca_select(tn_eps* coarse, const tn_mesh* coarsest, const void* flags, tn_stream_t s);
-/* get_local_matrix of a stored operator (kerngen.hpp:481-495), host out[18*18],
- * dimension-wise dof order (component*6 + wedge node); synchronous */
+/* GCACapable::get_local_matrix (kerngen.hpp:481-497), host out[18*18],
+ * dimension-wise dof order (component*6 + wedge node); synchronous.  Stored
+ * operators return the stored matrix; matrix-free ones assemble the wedge's
+ * Neumann matrix from the geometry and viscosity (assemble_local_matrix,
+ * kerngen.hpp:2051-2108; computed in FP64 for either operator dtype). */
 int tn_eps_get_local_matrix(const tn_eps* op, int sd, int x, int y, int r, int wedge, double* out);
+/* GCACapable::set_local_matrix (kerngen.hpp:468-479, operator.hpp:80-112): host in[18*18],
+ * same order; the operator must have stored matrices; storage is packed
+ * symmetric, so a nonsymmetric matrix (|A - A^T| > 1e-12 max|A|) is rejected.
+ * The boundary forms are rebuilt at the next apply.  Synchronous. */
+int tn_eps_set_local_matrix(tn_eps* op, int sd, int x, int y, int r, int wedge, const double* in);
+/* set_stored_matrix_mode(Full) (kerngen.hpp:443-464, local_matrix_storage.hpp:28-243):
+ * the operator switches to its stored-matrix path, every cell holding its own
+ * wedge matrices (those get_local_matrix assembles), stored in store_dtype. */
+int tn_eps_set_stored_matrices(tn_eps* op, int store_dtype, tn_stream_t s);
 
 /* ------------------------------------------------------------------ adjoint viscosity gradient
  * No counterpart in the reference (its viscosity is nodal, kerngen.hpp:94-96).

@@ -55,6 +55,7 @@ SIGNATURES = {
     "tn_sync": (i32, []),
     "tn_set_wait_limit": (i32, [i32]),
     "tn_mesh_create": (i32, [i32, i32, f64, f64, pvp]),
+    "tn_mesh_create_subdomains": (i32, [i32, i32, f64, f64, i32, i32, pvp]),
     "tn_mesh_create_radii": (i32, [i32, i32, C.POINTER(f64), pvp]),
     "tn_mesh_destroy": (i32, [vp]),
     "tn_mesh_dims": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]),
@@ -83,6 +84,8 @@ SIGNATURES = {
     "tn_eps_set_penalty_once": (i32, [vp, i32]),
     "tn_eps_gca_from": (i32, [vp, vp, i32, vp]),
     "tn_eps_get_local_matrix": (i32, [vp, i32, i32, i32, i32, i32, vp]),
+    "tn_eps_set_local_matrix": (i32, [vp, i32, i32, i32, i32, i32, vp]),
+    "tn_eps_set_stored_matrices": (i32, [vp, i32, vp]),
     "tn_eps_set_cell_eta": (i32, [vp, vp]),
     "tn_eps_gca_from_bracket": (i32, [vp, vp, i32, i32, vp]),
     "tn_eps_grad_log_cell_eta": (i32, [vp, vp, vp, vp, vp]),

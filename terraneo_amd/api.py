@@ -192,6 +192,14 @@ class Eps:
         check(lib().tn_eps_get_local_matrix(self.h, sd, x, y, r, w, out.ctypes.data))
         return out.reshape(18, 18)
 
+    def set_local_matrix(self, sd, x, y, r, w, mat):
+        a = np.ascontiguousarray(mat, dtype=np.float64).reshape(324)
+        check(lib().tn_eps_set_local_matrix(self.h, sd, x, y, r, w, a.ctypes.data))
+
+    def set_stored_matrices(self, store_dtype=TN_F64, stream=None):
+        """stored-matrix path with every cell's own matrices (set_stored_matrix_mode Full)"""
+        check(lib().tn_eps_set_stored_matrices(self.h, store_dtype, _stream_ptr(stream)))
+
     def set_penalty_epsilon(self, eps):
         check(lib().tn_eps_set_penalty_epsilon(self.h, eps))
 

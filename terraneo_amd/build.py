@@ -50,7 +50,30 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_driver(verbose)
     return LIB
+
+
+DRIVER_SRC = os.path.join(PKG, "..", "tests", "capi", "capi_driver.cpp")
+DRIVER = os.path.join(PKG, "..", "tests", "capi", "capi_driver")
+
+
+def build_driver(verbose=False):
+    """The C++ caller of the C ABI (tests/capi/capi_driver.cpp), header-only use
+    of include/terraneo_amd.h, linked against the library just built."""
+    if not os.path.exists(DRIVER_SRC):
+        return None
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "lib")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(PKG, "..", "include"), DRIVER_SRC,
+           "-o", DRIVER + ".tmp", "-L", PKG, "-lterraneo_amd", "-Wl,-rpath,$ORIGIN/../../terraneo_amd",
+           "-Wl,-rpath-link," + rocm_lib]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -48,6 +48,14 @@ void check_march_status()
 }  // namespace tn
 
 #define TN_API_BEGIN try {
+// closes a TN_API_BEGIN block without returning on success
+#define TN_API_END_SILENT           \
+    }                               \
+    catch (const std::exception& e) \
+    {                               \
+        tn::set_last_error(e.what()); \
+        return -1;                  \
+    }
 #define TN_API_END                          \
     }                                       \
     catch (const std::exception& e)         \
@@ -306,6 +314,17 @@ int tn_mesh_create(int lat_level, int rad_level, double r_min, double r_max, tn_
     radii[nr - 1] = r_max;
     *out = new_mesh(lat_level, radii);
     TN_API_END
+}
+
+int tn_mesh_create_subdomains(int lat_level, int rad_level, double r_min, double r_max, int s_lat, int s_rad,
+                              tn_mesh** out)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(s_lat >= 0 && s_rad >= 0, "subdomain refinement levels must be >= 0");
+    TN_REQUIRE(s_lat == 0 && s_rad == 0,
+               "diamonds split into subdomains (s_lat, s_rad > 0) are not supported: one subdomain per diamond");
+    TN_API_END_SILENT
+    return tn_mesh_create(lat_level, rad_level, r_min, r_max, out);
 }
 
 int tn_mesh_create_radii(int lat_level, int nr, const double* radii, tn_mesh** out)
@@ -700,32 +719,100 @@ int tn_eps_gca_select(tn_eps* coarse, const tn_mesh* coarsest, const void* flags
     TN_API_END
 }
 
-int tn_eps_get_local_matrix(const tn_eps* op, int sd, int x, int y, int r, int wedge, double* out)
+namespace {
+int64_t local_cell(const tn_eps* op, int sd, int x, int y, int r, int wedge)
 {
-    TN_API_BEGIN
-    TN_REQUIRE(op && out, "null pointer");
-    TN_REQUIRE(op->stored_dtype >= 0, "operator has no stored matrices");
     const auto& m = op->mesh->lvl;
     TN_REQUIRE(sd >= 0 && sd < 10 && x >= 0 && x < m.n - 1 && y >= 0 && y < m.n - 1 && r >= 0 && r < m.nr - 1 &&
                    (wedge == 0 || wedge == 1),
                "local matrix index out of range");
-    const int64_t cell = (((int64_t)sd * (m.n - 1) + x) * (m.n - 1) + y) * (m.nr - 1) + r;
-    const size_t es = op->stored_dtype == TN_F64 ? 8 : 4;
-    std::vector<double> packed(tn::PACKED);
-    for (int k = 0; k < tn::PACKED; ++k) {
-        const size_t off =
-            (size_t)(es == 8 ? tn::stored_at<double>(cell, wedge, k) : tn::stored_at<float>(cell, wedge, k)) * es;
-        if (es == 8) {
-            TN_HIP(hipMemcpy(&packed[k], op->stored.p + off, 8, hipMemcpyDeviceToHost));
-        } else {
-            float f;
-            TN_HIP(hipMemcpy(&f, op->stored.p + off, 4, hipMemcpyDeviceToHost));
-            packed[k] = f;
-        }
+    return (((int64_t)sd * (m.n - 1) + x) * (m.n - 1) + y) * (m.nr - 1) + r;
+}
+}  // namespace
+
+// GCACapable::get_local_matrix (epsilon_divdiv_kerngen.hpp:481-497): the stored
+// matrix when the operator has stored matrices, else the wedge's matrix
+// assembled from the geometry and the viscosity (assemble_local_matrix,
+// :2051-2108, the Neumann matrix without boundary masks); one bulk copy.
+int tn_eps_get_local_matrix(const tn_eps* op_c, int sd, int x, int y, int r, int wedge, double* out)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op_c && out, "null pointer");
+    tn_eps* op = const_cast<tn_eps*>(op_c);
+    const int64_t cell = local_cell(op, sd, x, y, r, wedge);
+    const MeshLevel& m = const_cast<tn_mesh*>(op->mesh)->lvl.dev();
+    if (op->lm.n != 324) op->lm.alloc(324);
+    const hipStream_t st = nullptr;
+    if (op->stored_dtype == TN_F64) {
+        tn::launch_stored_get<double>((const double*)op->stored.p, cell, wedge, op->lm.p, st);
+    } else if (op->stored_dtype == TN_F32) {
+        tn::launch_stored_get<float>((const float*)op->stored.p, cell, wedge, op->lm.p, st);
+    } else {
+        TN_REQUIRE(op->eta || op->eta_cell, "local matrix: viscosity not set");
+        if (op->dtype == TN_F64)
+            tn::launch_local_matrix<double>(m, (const double*)op->eta, (const double*)op->eta_cell, sd, x, y, r, wedge,
+                                            op->lm.p, st);
+        else
+            tn::launch_local_matrix<float>(m, (const float*)op->eta, (const float*)op->eta_cell, sd, x, y, r, wedge,
+                                           op->lm.p, st);
     }
-    int k = 0;
+    TN_HIP(hipMemcpy(out, op->lm.p, sizeof(double) * 324, hipMemcpyDeviceToHost));
+    TN_API_END
+}
+
+// GCACapable::set_local_matrix (kerngen.hpp:468-479): needs stored matrices.
+// The storage is packed symmetric, so the matrix must be symmetric.
+int tn_eps_set_local_matrix(tn_eps* op, int sd, int x, int y, int r, int wedge, const double* in)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op && in, "null pointer");
+    TN_REQUIRE(op->stored_dtype >= 0,
+               "set_local_matrix: the operator has no stored matrices (tn_eps_set_stored_matrices or GCA first)");
+    const int64_t cell = local_cell(op, sd, x, y, r, wedge);
+    double amax = 0.0, asym = 0.0;
     for (int i = 0; i < 18; ++i)
-        for (int j = i; j < 18; ++j, ++k) out[i * 18 + j] = out[j * 18 + i] = packed[k];
+        for (int j = 0; j < 18; ++j) {
+            amax = std::max(amax, std::fabs(in[i * 18 + j]));
+            asym = std::max(asym, std::fabs(in[i * 18 + j] - in[j * 18 + i]));
+        }
+    TN_REQUIRE(asym <= 1e-12 * amax, "set_local_matrix: matrix is not symmetric (storage is packed symmetric)");
+    if (op->lm.n != 324) op->lm.alloc(324);
+    TN_HIP(hipMemcpy(op->lm.p, in, sizeof(double) * 324, hipMemcpyHostToDevice));
+    if (op->stored_dtype == TN_F64)
+        tn::launch_stored_set<double>((double*)op->stored.p, cell, wedge, op->lm.p, nullptr);
+    else
+        tn::launch_stored_set<float>((float*)op->stored.p, cell, wedge, op->lm.p, nullptr);
+    TN_HIP(hipDeviceSynchronize());
+    op->blk_ready = false;  // boundary forms follow the matrices
+    TN_API_END
+}
+
+// set_stored_matrix_mode(Full) (kerngen.hpp:443-464, local_matrix_storage.hpp:28-243):
+// switch the operator to its stored-matrix path with every cell's own
+// matrices (the on-the-fly assembly of all cells), stored in store_dtype.
+int tn_eps_set_stored_matrices(tn_eps* op, int store_dtype, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(op, "null operator");
+    TN_REQUIRE(store_dtype == TN_F64 || store_dtype == TN_F32, "store dtype must be TN_F64 or TN_F32");
+    TN_REQUIRE(op->eta || op->eta_cell, "stored matrices: viscosity not set");
+    const MeshLevel& m = const_cast<tn_mesh*>(op->mesh)->lvl.dev();
+    const int64_t bytes = tn::stored_elems(op->cells()) * (store_dtype == TN_F64 ? 8 : 4);
+    op->stored.alloc((size_t)bytes);
+    op->stored_dtype = store_dtype;
+    op->blk_ready = false;
+    auto go = [&](auto* eta, auto* stored) {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(eta)>>;
+        using S = std::remove_pointer_t<decltype(stored)>;
+        tn::launch_stored_own<T, S>(m, eta, static_cast<const T*>(op->eta_cell), stored, (hipStream_t)s);
+    };
+    if (op->dtype == TN_F64) {
+        if (store_dtype == TN_F64) go((const double*)op->eta, (double*)op->stored.p);
+        else go((const double*)op->eta, (float*)op->stored.p);
+    } else {
+        if (store_dtype == TN_F64) go((const float*)op->eta, (double*)op->stored.p);
+        else go((const float*)op->eta, (float*)op->stored.p);
+    }
     TN_API_END
 }
 

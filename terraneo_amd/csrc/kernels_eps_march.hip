@@ -675,8 +675,17 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
                     if (mode != TN_NEUMANN || pen) {
                         const int64_t lat = ((int64_t)d * n + x) * n + qy;
                         if (mode != TN_NEUMANN) {
-                            const T* xk = xkeep[w][qy & 1][side];
-                            const V3<T> xv{xk[0], xk[1], xk[2]};
+                            // the node's raw source: kept when its column was loaded as a corner;
+                            // the last node row (no quads in this wave's strip) reads it
+                            V3<T> xv;
+                            if (strip_ok) {
+                                const T* xk = xkeep[w][qy & 1][side];
+                                xv = {xk[0], xk[1], xk[2]};
+                            } else {
+                                const uint32_t lo = (uint32_t)level * SZ;
+                                xv = {buf_ld(rs_src, lo, cb, T(0)), buf_ld(rs_src, lo, cb + Nb, T(0)),
+                                      buf_ld(rs_src, lo, cb + 2 * Nb, T(0))};
+                            }
                             const T* blk = blk_tab + ((int64_t)side * 10 * n * n + lat) * 3;
                             v = test_value<T, false>(v, xv, mode, coords, lat, blk);
                         }
@@ -855,8 +864,8 @@ int eps_kernel_variant()
 {
     static const int v = [] {
         const char* e = std::getenv("TN_EPS_KERNEL");
-        const int k = e ? std::atoi(e) : 0;
-        return (k >= 0 && k <= 2) ? k : 0;
+        const int k = e ? std::atoi(e) : 1;
+        return (k >= 0 && k <= 2) ? k : 1;
     }();
     return v;
 }

@@ -518,8 +518,8 @@ __global__ void k_gca_collect(const T* __restrict__ k, int n, int nr, int levels
 // One thread per (cell, wedge, component block q <= p), as k_gca_build.
 template <typename T, typename S>
 __global__ void k_gca_select(const T* __restrict__ flags, int n0, int nr0, int levels, const double* __restrict__ geo,
-                             const double* __restrict__ radii, const T* __restrict__ eta, int n, int nr,
-                             S* __restrict__ stored)
+                             const double* __restrict__ radii, const T* __restrict__ eta,
+                             const T* __restrict__ eta_cell, int n, int nr, S* __restrict__ stored)
 {
     constexpr int QP[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
     const int ncl = n - 1, ncr = nr - 1;
@@ -535,12 +535,17 @@ __global__ void k_gca_select(const T* __restrict__ flags, int n0, int nr0, int l
     lat /= ncl;
     const int x = (int)(lat % ncl);
     const int d = (int)(lat / ncl);
-    if (flags[(((int64_t)d * n0 + (x >> levels)) * n0 + (y >> levels)) * nr0 + (r >> levels)] != T(0)) return;
+    if (flags && flags[(((int64_t)d * n0 + (x >> levels)) * n0 + (y >> levels)) * nr0 + (r >> levels)] != T(0)) return;
     const int q = QP[blk][0], p = QP[blk][1];
     const double* geo_w = geo + ((((int64_t)d * ncl + x) * ncl + y) * 2 + w) * GEO_STRIDE;
     double ksum = 0.0;
-    for (int j = 0; j < 6; ++j)
-        ksum += (double)eta[(((int64_t)d * n + x + WOFF[w][j][0]) * n + y + WOFF[w][j][1]) * nr + r + WOFF[w][j][2]];
+    if (eta_cell) {
+        ksum = 6.0 * (double)eta_cell[cell];
+    } else {
+        for (int j = 0; j < 6; ++j)
+            ksum +=
+                (double)eta[(((int64_t)d * n + x + WOFF[w][j][0]) * n + y + WOFF[w][j][1]) * nr + r + WOFF[w][j][2]];
+    }
     double A[6][6];
     eps_block_geo<double>(geo_w, radii[r], radii[r + 1], ksum, q, p, A);
     for (int i = 0; i < 6; ++i)
@@ -920,9 +925,100 @@ void launch_gca_select(const MeshLevel& coarsest, int levels, const T* flags, co
 {
     const int64_t total = 12LL * 10 * (m.n - 1) * (m.n - 1) * (m.nr - 1);
     hipLaunchKernelGGL((k_gca_select<T, S>), dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, flags,
-                       coarsest.n, coarsest.nr, levels, m.d_geo.p, m.d_radii.p, eta, m.n, m.nr, stored);
+                       coarsest.n, coarsest.nr, levels, m.d_geo.p, m.d_radii.p, eta, (const T*)nullptr, m.n, m.nr,
+                       stored);
     TN_HIP(hipGetLastError());
 }
+
+template <typename T, typename S>
+void launch_stored_own(const MeshLevel& m, const T* eta, const T* eta_cell, S* stored, hipStream_t s)
+{
+    const int64_t total = 12LL * 10 * (m.n - 1) * (m.n - 1) * (m.nr - 1);
+    hipLaunchKernelGGL((k_gca_select<T, S>), dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s,
+                       (const T*)nullptr, 0, 0, 0, m.d_geo.p, m.d_radii.p, eta, eta_cell, m.n, m.nr, stored);
+    TN_HIP(hipGetLastError());
+}
+
+namespace {
+// one wedge's 18x18 Neumann matrix from the geometry (dimension-wise dof
+// order component * 6 + node), a thread per 6x6 component block
+template <typename T>
+__global__ void k_local_matrix(const double* __restrict__ geo, const double* __restrict__ radii,
+                               const T* __restrict__ eta, const T* __restrict__ eta_cell, int n, int nr, int d, int x,
+                               int y, int r, int w, double* __restrict__ out)
+{
+    const int blk = threadIdx.x;
+    if (blk >= 9) return;
+    const int q = blk / 3, p = blk % 3, ncl = n - 1, ncr = nr - 1;
+    const int64_t cell = (((int64_t)d * ncl + x) * ncl + y) * ncr + r;
+    const double* geo_w = geo + ((((int64_t)d * ncl + x) * ncl + y) * 2 + w) * GEO_STRIDE;
+    double ksum = 0.0;
+    if (eta_cell) {
+        ksum = 6.0 * (double)eta_cell[cell];
+    } else {
+        for (int j = 0; j < 6; ++j)
+            ksum +=
+                (double)eta[(((int64_t)d * n + x + WOFF[w][j][0]) * n + y + WOFF[w][j][1]) * nr + r + WOFF[w][j][2]];
+    }
+    double A[6][6];
+    eps_block_geo<double>(geo_w, radii[r], radii[r + 1], ksum, q, p, A);
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) out[(i + 6 * q) * 18 + j + 6 * p] = A[i][j];
+}
+
+// full 18x18 matrix of one stored wedge <-> a dense buffer (packed symmetric storage)
+template <typename S>
+__global__ void k_stored_get(const S* __restrict__ stored, int64_t cell, int w, double* __restrict__ out)
+{
+    const int t = threadIdx.x;
+    if (t >= 324) return;
+    const int i = t / 18, j = t % 18;
+    out[t] = (double)stored[stored_at<S>(cell, w, i <= j ? pk(i, j) : pk(j, i))];
+}
+template <typename S>
+__global__ void k_stored_set(S* __restrict__ stored, int64_t cell, int w, const double* __restrict__ in)
+{
+    const int t = threadIdx.x;
+    if (t >= 324) return;
+    const int i = t / 18, j = t % 18;
+    if (i <= j) stored[stored_at<S>(cell, w, pk(i, j))] = (S)in[t];
+}
+}  // namespace
+
+template <typename T>
+void launch_local_matrix(const MeshLevel& m, const T* eta, const T* eta_cell, int d, int x, int y, int r, int w,
+                         double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_local_matrix<T>, dim3(1), dim3(64), 0, s, m.d_geo.p, m.d_radii.p, eta, eta_cell, m.n, m.nr,
+                       d, x, y, r, w, out);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename S>
+void launch_stored_get(const S* stored, int64_t cell, int w, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stored_get<S>, dim3(1), dim3(384), 0, s, stored, cell, w, out);
+    TN_HIP(hipGetLastError());
+}
+
+template <typename S>
+void launch_stored_set(S* stored, int64_t cell, int w, const double* in, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stored_set<S>, dim3(1), dim3(384), 0, s, stored, cell, w, in);
+    TN_HIP(hipGetLastError());
+}
+template void launch_local_matrix<double>(const MeshLevel&, const double*, const double*, int, int, int, int, int,
+                                          double*, hipStream_t);
+template void launch_local_matrix<float>(const MeshLevel&, const float*, const float*, int, int, int, int, int,
+                                         double*, hipStream_t);
+template void launch_stored_get<double>(const double*, int64_t, int, double*, hipStream_t);
+template void launch_stored_get<float>(const float*, int64_t, int, double*, hipStream_t);
+template void launch_stored_set<double>(double*, int64_t, int, const double*, hipStream_t);
+template void launch_stored_set<float>(float*, int64_t, int, const double*, hipStream_t);
+template void launch_stored_own<double, double>(const MeshLevel&, const double*, const double*, double*, hipStream_t);
+template void launch_stored_own<double, float>(const MeshLevel&, const double*, const double*, float*, hipStream_t);
+template void launch_stored_own<float, double>(const MeshLevel&, const float*, const float*, double*, hipStream_t);
+template void launch_stored_own<float, float>(const MeshLevel&, const float*, const float*, float*, hipStream_t);
 
 template <typename T, typename S, typename SF>
 void launch_gca_build(const GcaArgs<T, S, SF>& a, bool fine_geo, hipStream_t s)

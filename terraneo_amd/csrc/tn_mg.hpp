@@ -119,6 +119,18 @@ void launch_gca_collect(const MeshLevel& fine, const MeshLevel& coarsest, int le
 template <typename T, typename S>
 void launch_gca_select(const MeshLevel& coarsest, int levels, const T* flags, const MeshLevel& m, const T* eta,
                        S* stored, hipStream_t s);
+// every cell's own (rediscretized) wedge matrices into the packed storage
+template <typename T, typename S>
+void launch_stored_own(const MeshLevel& m, const T* eta, const T* eta_cell, S* stored, hipStream_t s);
+// one wedge's 18x18 Neumann matrix computed from the geometry into out[324] (device)
+template <typename T>
+void launch_local_matrix(const MeshLevel& m, const T* eta, const T* eta_cell, int d, int x, int y, int r, int w,
+                         double* out, hipStream_t s);
+// one stored wedge matrix to / from a dense device buffer of 324 doubles
+template <typename S>
+void launch_stored_get(const S* stored, int64_t cell, int w, double* out, hipStream_t s);
+template <typename S>
+void launch_stored_set(S* stored, int64_t cell, int w, const double* in, hipStream_t s);
 template <typename T, typename S>
 void launch_stored_apply(const StoredArgs<T, S>& a, bool diagonal, hipStream_t s);
 // elements of StoredArgs::csum + ctop for an n x n x nr mesh

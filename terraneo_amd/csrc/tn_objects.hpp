@@ -28,6 +28,8 @@ struct tn_eps {
     // stored apply: per-cell corner sums between its two passes (12 per cell
     // plus 12 per radial chunk top), allocated on first use
     tn::DevBuf<unsigned char> cellsum;
+    // one 18x18 local matrix in transit (get / set_local_matrix), allocated on first use
+    tn::DevBuf<double> lm;
     int64_t cells() const
     {
         const auto& m = mesh->lvl;

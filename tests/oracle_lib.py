@@ -12,6 +12,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+PORT_SO = os.path.join(ORACLE_DIR, "_build", "libcpuport.so")
 
 NEUMANN, DIRICHLET, FREESLIP = 0, 1, 2
 
@@ -20,6 +21,33 @@ _lib = None
 
 def build():
     subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+
+
+_port = None
+
+
+def cpu_port():
+    """the benchmark's CPU baseline (oracle/cpu_port.c): matrix-free OpenMP apply"""
+    global _port
+    if _port is None:
+        if not os.path.exists(PORT_SO):
+            build()
+        L = C.CDLL(PORT_SO)
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.cpu_eps_apply.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp, C.c_int, C.c_int, ip, C.c_int]
+        _port = L
+    return _port
+
+
+def cpu_port_apply(mesh, eta, x, bc_cmb, bc_surf, y=None):
+    """y = A x by the CPU baseline port (Dirichlet / Neumann shells)"""
+    y = np.zeros_like(x) if y is None else y
+    cpu_port().cpu_eps_apply(mesh.n, mesh.nr, np.ascontiguousarray(mesh.coords.reshape(-1)), mesh.radii,
+                             np.ascontiguousarray(eta, dtype=np.float64).reshape(-1),
+                             np.ascontiguousarray(x, dtype=np.float64).reshape(-1), y, bc_cmb, bc_surf,
+                             np.ascontiguousarray(mesh.groups.reshape(-1)), mesh.ngroups)
+    return y
 
 
 def lib():

@@ -41,3 +41,18 @@ def test_error_path_without_device():
     rc = tn.lib().tn_mesh_create(3, 0, 0.5, 1.0, ctypes.byref(h))
     assert rc != 0
     assert b"radial level" in tn.lib().tn_last_error()
+
+
+def test_mesh_subdomain_split_rejected():
+    import terraneo_amd as tn
+
+    h = ctypes.c_void_p()
+    rc = tn.lib().tn_mesh_create_subdomains(3, 3, 0.55, 1.0, 1, 0, ctypes.byref(h))
+    assert rc != 0
+    assert b"subdomains" in tn.lib().tn_last_error()
+    rc = tn.lib().tn_mesh_create_subdomains(3, 3, 0.55, 1.0, 0, 0, ctypes.byref(h))
+    assert rc == 0
+    n, nr, N = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+    assert tn.lib().tn_mesh_dims(h, ctypes.byref(n), ctypes.byref(nr), ctypes.byref(N)) == 0
+    assert (n.value, nr.value) == (9, 9)
+    tn.lib().tn_mesh_destroy(h)
