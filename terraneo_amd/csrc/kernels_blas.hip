@@ -279,7 +279,9 @@ __global__ __launch_bounds__(256) void k_penalty_moment(const T* __restrict__ u,
     }
 }
 
-// beta[k] = sum over the nblk partial sums (one block)
+// beta[k] = sum over the nblk partial sums (one block); then the penalty's
+// rotation vector gamma = S beta into beta[13..16) (S = beta[4..13), row-major),
+// which the apply kernels read instead of forming it per node
 __global__ __launch_bounds__(256) void k_penalty_finalize(const double* __restrict__ partial, int nblk,
                                                           double* __restrict__ beta)
 {
@@ -287,10 +289,16 @@ __global__ __launch_bounds__(256) void k_penalty_finalize(const double* __restri
     for (int b = threadIdx.x; b < nblk; b += blockDim.x)
 #pragma unroll
         for (int k = 0; k < 3; ++k) m[k] += partial[3 * b + k];
+    double bk[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double t = block_total(m[k]);
-        if (threadIdx.x == 0) beta[k] = t;
+        bk[k] = block_total(m[k]);
+        if (threadIdx.x == 0) beta[k] = bk[k];
+    }
+    if (threadIdx.x == 0) {
+        const double* S = beta + 4;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) beta[13 + i] = S[3 * i] * bk[0] + S[3 * i + 1] * bk[1] + S[3 * i + 2] * bk[2];
     }
 }
 

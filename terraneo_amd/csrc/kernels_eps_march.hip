@@ -437,7 +437,7 @@ __device__ __forceinline__ void wedge_folded(const T U1[3], const T U2[3], const
 // Neumann cell from its four corner sums (0 (x,y), 1 (x+1,y), 2 (x,y+1),
 // 3 (x+1,y+1); wedges (0,1,2) and (3,2,1), kernel_helpers.hpp:302-327).
 // hd6 = (dr/2)/6; ecell6 = 6 eta_cell, or a negative value for the nodal viscosity.
-template <typename T>
+template <typename T, bool CELL>
 __device__ __forceinline__ void cell_folded(const CornerSum<T>& c0, const CornerSum<T>& c1, const CornerSum<T>& c2,
                                             const CornerSum<T>& c3, const T* __restrict__ geo, T hd6, T gam,
                                             T ecell6, T yb[4][3], T yt[4][3])
@@ -448,7 +448,10 @@ __device__ __forceinline__ void cell_folded(const CornerSum<T>& c0, const Corner
     const T e12 = c1.e + c2.e;
     T p1a[3], p2a[3], qa[3], p1b[3], p2b[3], qb[3];
     {
-        const T* g = geo;
+        // the wedge's geometry (wave-uniform: scalar registers)
+        T g[GEO_STRIDE];
+#pragma unroll
+        for (int i = 0; i < GEO_STRIDE; ++i) g[i] = geo[i];
         T U1[3], U2[3], D[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -456,11 +459,16 @@ __device__ __forceinline__ void cell_folded(const CornerSum<T>& c0, const Corner
             U2[k] = c2.S[k] - c0.S[k];
             D[k] = c0.d[k] + d12[k];
         }
-        const T es = ecell6 >= T(0) ? ecell6 : c0.e + e12;
+        const T es = CELL ? ecell6 : c0.e + e12;
         wedge_folded(U1, U2, D, es * (hd6 * g[9]), g, p1a, p2a, qa);
     }
+    // the second wedge's geometry is read only now, so the two wedges'
+    // tables do not occupy scalar registers at the same time
+    asm volatile("" ::: "memory");
     {
-        const T* g = geo + GEO_STRIDE;
+        T g[GEO_STRIDE];
+#pragma unroll
+        for (int i = 0; i < GEO_STRIDE; ++i) g[i] = geo[GEO_STRIDE + i];
         T U1[3], U2[3], D[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -468,7 +476,7 @@ __device__ __forceinline__ void cell_folded(const CornerSum<T>& c0, const Corner
             U2[k] = c1.S[k] - c3.S[k];
             D[k] = c3.d[k] + d12[k];
         }
-        const T es = ecell6 >= T(0) ? ecell6 : c3.e + e12;
+        const T es = CELL ? ecell6 : c3.e + e12;
         wedge_folded(U1, U2, D, es * (hd6 * g[9]), g, p1b, p2b, qb);
     }
 #pragma unroll
@@ -502,12 +510,19 @@ __device__ __forceinline__ double shift_up1(double v)
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-template <typename T, int W, bool PF>
+// CFG bits: 1 a shell is not Neumann (boundary lanes), 2 fs/fs rotation
+// penalty, 4 per-cell viscosity; each case is compiled on its own so the
+// common ones carry no dead state in registers
+template <typename T, int W, int CFG>
 __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __restrict__ geo_tab,
                                                      const T* __restrict__ coords, const T* __restrict__ blk_tab,
                                                      const double* __restrict__ coords_d,
                                                      const double* __restrict__ pen)
 {
+    constexpr bool BND = (CFG & 1) != 0, PEN = (CFG & 2) != 0, CELL = (CFG & 4) != 0;
+    // PF: a step's two new corners are loaded during the step before (their
+    // latency is then covered by that step's arithmetic; more registers)
+    constexpr bool PF = (CFG & 8) != 0;
     constexpr int OWN = W - 1;
     constexpr int RING = (sizeof(T) == 8 && W > 12) ? 3 : 4;
     constexpr uint32_t SZ = sizeof(T);
@@ -533,7 +548,7 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
     const uint32_t Nb = Nn * SZ;
     const __amdgpu_buffer_rsrc_t rs_src = buf_rsrc(a.src, 3 * Nb);
     const __amdgpu_buffer_rsrc_t rs_dst = buf_rsrc(a.dst, 3 * Nb);
-    const bool nodal = a.eta_cell == nullptr;
+    constexpr bool nodal = !CELL;
     const __amdgpu_buffer_rsrc_t rs_eta = buf_rsrc(nodal ? (const void*)a.eta : (const void*)a.src, nodal ? Nb : 0u);
     const int bmode = a.bc_cmb, tmode = a.bc_surf;
     const int nchunks = (ncr + 63) >> 6;
@@ -556,9 +571,9 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
         const T half_dr = T(0.5) * (r1 - r0), r_mid = T(0.5) * (r0 + r1);
         const T gam = r_mid / half_dr, hd6 = half_dr * T(1.0 / 6.0);
         // boundary lanes of this chunk (trial side on load, test side at the store)
-        const bool bot_b = h == 0 && bmode != TN_NEUMANN && lane == 0;
-        const bool top_b = last && tmode != TN_NEUMANN && c == ncr - 1;
-        const bool any_b = (h == 0 && bmode != TN_NEUMANN) || (last && tmode != TN_NEUMANN);
+        const bool bot_b = BND && h == 0 && bmode != TN_NEUMANN && lane == 0;
+        const bool top_b = BND && last && tmode != TN_NEUMANN && c == ncr - 1;
+        const bool any_b = BND && ((h == 0 && bmode != TN_NEUMANN) || (last && tmode != TN_NEUMANN));
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             carry[w][0][j][lane] = T(0);
@@ -584,65 +599,60 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
             }
             make_corner(cs, r, gam);
         };
+        // Steps of this segment: quads qy = qs .. y1 - 1 of the strip, with
+        // qs = y0 - 1 (the column before the segment, computed again for its
+        // carried sums) or 0 when the segment starts the diamond (the step
+        // before it would carry zeros only; every wave of the block has the
+        // same y0, so all of them skip it).  Quads exist for qy <= n - 2; the
+        // step qy = n - 1 only finishes the last column.  Waves whose strip
+        // has no quads (x = -1, x = n - 1) take empty steps throughout.
+        const int qs = y0 == 0 ? 0 : y0 - 1;
+        const int goff0 = ((d * ncl + x) * ncl + qs) * 2 * GEO_STRIDE;
+        const int step0 = step;
+        const int qe = min(y1 - 1, n - 2);  // last quad of the segment
         CornerSum<T> C0, C1, N0, N1;
-        CornerRawCol<T> R0, R1;
-        {
-            const int yp = y0 - 1;
-            if (strip_ok && yp >= 0) {
-                const uint32_t cb = colx + (uint32_t)yp * colstep;
-                load_corner(R0, rs_src, rs_eta, cb, Nb, voff, nodal);
-                load_corner(R1, rs_src, rs_eta, cb + rowstep, Nb, voff, nodal);
-                process(C0, R0, x, yp, true);
-                process(C1, R1, x + 1, yp, false);
-            } else {
-#pragma unroll
-                for (int j = 0; j < 3; ++j) C0.S[j] = C0.d[j] = C1.S[j] = C1.d[j] = T(0);
-                C0.e = C1.e = T(0);
-            }
-            if (PF && strip_ok && y0 <= n - 1) {
-                const uint32_t cb = colx + (uint32_t)y0 * colstep;
-                load_corner(R0, rs_src, rs_eta, cb, Nb, voff, nodal);
-                load_corner(R1, rs_src, rs_eta, cb + rowstep, Nb, voff, nodal);
+        CornerRawCol<T> R0, R1;  // loaded corner columns (with PF: those of the next step)
+        if (strip_ok) {
+            const uint32_t cb = colx + (uint32_t)qs * colstep;
+            load_corner(R0, rs_src, rs_eta, cb, Nb, voff, nodal);
+            load_corner(R1, rs_src, rs_eta, cb + rowstep, Nb, voff, nodal);
+            process(C0, R0, x, qs, true);
+            process(C1, R1, x + 1, qs, false);
+            if (PF && qs <= qe) {
+                load_corner(R0, rs_src, rs_eta, cb + colstep, Nb, voff, nodal);
+                load_corner(R1, rs_src, rs_eta, cb + colstep + rowstep, Nb, voff, nodal);
             }
         }
-        int goff = ((d * ncl + x) * ncl + (y0 - 1)) * 2 * GEO_STRIDE;
 
-        for (int qy = y0 - 1; qy <= y1 - 1; ++qy, ++step, goff += 2 * GEO_STRIDE) {
-            const bool quad_ok = strip_ok && qy >= 0 && qy <= n - 2;
-            const bool next_ok = strip_ok && qy + 1 <= n - 1;
+        // one step: quad (x, qy) from its corners (C0, C1) at qy and (N0, N1) at
+        // qy + 1, which are loaded here (quad = true); quad = false: an empty
+        // step (no quad, nothing loaded).  The loop below alternates the roles
+        // of the two pairs, so no corner data is copied between steps.
+        auto body = [&](const int qy, CornerSum<T>& C0, CornerSum<T>& C1, CornerSum<T>& N0, CornerSum<T>& N1,
+                        const bool quad) __attribute__((always_inline)) {
+            const int step = step0 + (qy - qs);
             T(*E)[65] = ebuf[step % RING][w];
-            if (next_ok) {
+            T yb[4][3], yt[4][3];
+            if (quad) {
+                const uint32_t cb = colx + (uint32_t)(qy + 1) * colstep;
                 if (!PF) {
-                    const uint32_t cb = colx + (uint32_t)(qy + 1) * colstep;
                     load_corner(R0, rs_src, rs_eta, cb, Nb, voff, nodal);
                     load_corner(R1, rs_src, rs_eta, cb + rowstep, Nb, voff, nodal);
                 }
                 process(N0, R0, x, qy + 1, true);
                 process(N1, R1, x + 1, qy + 1, false);
-                if (PF && qy + 2 <= n - 1 && qy + 1 <= y1 - 1) {
-                    const uint32_t cb = colx + (uint32_t)(qy + 2) * colstep;
-                    load_corner(R0, rs_src, rs_eta, cb, Nb, voff, nodal);
-                    load_corner(R1, rs_src, rs_eta, cb + rowstep, Nb, voff, nodal);
+                if (PF && qy + 1 <= qe) {
+                    load_corner(R0, rs_src, rs_eta, cb + colstep, Nb, voff, nodal);
+                    load_corner(R1, rs_src, rs_eta, cb + colstep + rowstep, Nb, voff, nodal);
                 }
-            }
-            T yb[4][3], yt[4][3];
-            if (quad_ok) {
-                const T* gsrc = geo_tab + __builtin_amdgcn_readfirstlane(goff);
-                T geo[2 * GEO_STRIDE];
-#pragma unroll
-                for (int i = 0; i < 2 * GEO_STRIDE; ++i) geo[i] = gsrc[i];
-                const T ecell6 =
-                    nodal ? T(-1) : T(6) * a.eta_cell[(((int64_t)d * ncl + x) * ncl + qy) * ncr + cc];
-                cell_folded(C0, C1, N0, N1, geo, hd6, gam, ecell6, yb, yt);
+                const T* gsrc = geo_tab + __builtin_amdgcn_readfirstlane(goff0 + (qy - qs) * 2 * GEO_STRIDE);
+                const T ecell6 = CELL ? T(6) * a.eta_cell[(((int64_t)d * ncl + x) * ncl + qy) * ncr + cc] : T(0);
+                cell_folded<T, CELL>(C0, C1, N0, N1, gsrc, hd6, gam, ecell6, yb, yt);
             } else {
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
 #pragma unroll
                     for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
-            }
-            if (next_ok) {
-                C0 = N0;
-                C1 = N1;
             }
             if (w + 1 < W) march::wait_counter(&used[w + 1], step - RING + 1, a.guard);
             T part[3], tp[3] = {0, 0, 0};
@@ -672,9 +682,9 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
                 const uint32_t cb = colx + (uint32_t)qy * colstep;
                 const int slot = qy - y0;
                 auto finish = [&](V3<T> v, int level, int mode, int side) {
-                    if (mode != TN_NEUMANN || pen) {
+                    if ((BND && mode != TN_NEUMANN) || PEN) {
                         const int64_t lat = ((int64_t)d * n + x) * n + qy;
-                        if (mode != TN_NEUMANN) {
+                        if (BND && mode != TN_NEUMANN) {
                             // the node's raw source: kept when its column was loaded as a corner;
                             // the last node row (no quads in this wave's strip) reads it
                             V3<T> xv;
@@ -689,13 +699,10 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
                             const T* blk = blk_tab + ((int64_t)side * 10 * n * n + lat) * 3;
                             v = test_value<T, false>(v, xv, mode, coords, lat, blk);
                         }
-                        if (pen) {
-                            // fs/fs rotation penalty y += gamma x p, gamma = S beta (FP64)
-                            const double* bt = pen;
-                            const double* S = pen + 4;
-                            const double g0 = S[0] * bt[0] + S[1] * bt[1] + S[2] * bt[2];
-                            const double g1 = S[3] * bt[0] + S[4] * bt[1] + S[5] * bt[2];
-                            const double g2 = S[6] * bt[0] + S[7] * bt[1] + S[8] * bt[2];
+                        if (PEN) {
+                            // fs/fs rotation penalty y += gamma x p, gamma = S beta (FP64, formed
+                            // by the moment pass's finalize)
+                            const double g0 = pen[13], g1 = pen[14], g2 = pen[15];
                             const double r = a.radii_d[level];
                             const double px = coords_d[3 * lat] * r, py = coords_d[3 * lat + 1] * r,
                                          pz = coords_d[3 * lat + 2] * r;
@@ -735,7 +742,19 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
                 }
             }
             march::post_counter(&used[w], step + 1, lane);
+        };
+        if (strip_ok) {
+            int qy = qs;
+            for (; qy + 1 <= qe; qy += 2) {
+                body(qy, C0, C1, N0, N1, true);
+                body(qy + 1, N0, N1, C0, C1, true);
+            }
+            if (qy <= qe) body(qy++, C0, C1, N0, N1, true);
+            if (qy <= y1 - 1) body(qy, C0, C1, N0, N1, false);  // qy = n - 1
+        } else {
+            for (int qy = qs; qy <= y1 - 1; ++qy) body(qy, C0, C1, N0, N1, false);
         }
+        step += y1 - qs;
     }
 }
 
@@ -858,8 +877,8 @@ int march_block_rows(int dtype_size) { return (dtype_size == 8 ? march_waves<dou
 int march_waves_per_cu(int dtype_size) { return march_block_rows(dtype_size) + 1; }
 
 // matrix-apply kernel variant: 0 the per-quad-load march, 1 the carried-corner
-// march, 2 the same with the next step's loads issued one step ahead
-// (TN_EPS_KERNEL overrides; tuning)
+// march, 2 the same with each step's loads issued a step ahead (TN_EPS_KERNEL
+// overrides; tuning)
 int eps_kernel_variant()
 {
     static const int v = [] {
@@ -876,15 +895,29 @@ void launch_march_w(const MarchArgs<T>& a, bool diagonal, hipStream_t s)
     constexpr int OWN = W - 1;
     const int nxb = (a.n + OWN - 1) / OWN, nys = (a.n + a.seg - 1) / a.seg;
     const dim3 gr((unsigned)(10 * nxb * nys)), bl(64 * W);
-    const int var = eps_kernel_variant();
-    if (diagonal)
+    // the carried-corner kernel is built for 8, 12 and 16 waves per block
+    const int var = W >= 8 ? eps_kernel_variant() : 0;
+    if (diagonal) {
         hipLaunchKernelGGL((k_eps_march<T, true, W>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen);
-    else if (var == 1)
-        hipLaunchKernelGGL((k_eps_flow<T, W, false>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen);
-    else if (var == 2)
-        hipLaunchKernelGGL((k_eps_flow<T, W, true>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen);
-    else
+    } else if (var == 0) {
         hipLaunchKernelGGL((k_eps_march<T, false, W>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen);
+    } else {
+        if constexpr (W >= 8) {
+            const int cfg = ((a.bc_cmb != TN_NEUMANN || a.bc_surf != TN_NEUMANN) ? 1 : 0) | (a.pen ? 2 : 0) |
+                            (a.eta_cell ? 4 : 0) | (var == 2 ? 8 : 0);
+            TN_REQUIRE(3ULL * 10 * a.n * a.n * a.nr * sizeof(T) < (1ULL << 32),
+                       "apply: field larger than 4 GB (buffer resources address 32-bit byte offsets)");
+            switch (cfg) {
+#define TN_FLOW(C)                                                                                                \
+    case C:                                                                                                      \
+        hipLaunchKernelGGL((k_eps_flow<T, W, C>), gr, bl, 0, s, a, a.geo, a.coords, a.blk, a.coords_d, a.pen); \
+        break;
+                TN_FLOW(0) TN_FLOW(1) TN_FLOW(2) TN_FLOW(3) TN_FLOW(4) TN_FLOW(5) TN_FLOW(6) TN_FLOW(7)
+                TN_FLOW(8) TN_FLOW(9) TN_FLOW(10) TN_FLOW(11) TN_FLOW(12) TN_FLOW(13) TN_FLOW(14) TN_FLOW(15)
+#undef TN_FLOW
+            }
+        }
+    }
     TN_HIP(hipGetLastError());
 }
 

@@ -760,11 +760,8 @@ __global__ __launch_bounds__(256) void k_stored_gather(StoredArgs<T, S> a)
         }
     }
     if (a.pen) {
-        const double* bt = a.pen;
-        const double* Sm = a.pen + 4;
-        const double g0 = Sm[0] * bt[0] + Sm[1] * bt[1] + Sm[2] * bt[2];
-        const double g1 = Sm[3] * bt[0] + Sm[4] * bt[1] + Sm[5] * bt[2];
-        const double g2 = Sm[6] * bt[0] + Sm[7] * bt[1] + Sm[8] * bt[2];
+        // gamma = S beta, formed by the moment pass's finalize
+        const double g0 = a.pen[13], g1 = a.pen[14], g2 = a.pen[15];
         const double r = a.radii[lev];
         const double px = a.pen_coords[3 * lat] * r, py = a.pen_coords[3 * lat + 1] * r,
                      pz = a.pen_coords[3 * lat + 2] * r;

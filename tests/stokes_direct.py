@@ -153,3 +153,36 @@ def solve_assess(kind, level, penalty_eps=1.0):
     ev = np.sqrt(np.sum((u - ue) ** 2) / nu)
     ep = np.sqrt(np.sum((p - pe) ** 2) / npr)
     return ev, ep
+
+
+def solve_freeslip_penalized(mv, mp, eps_op, bu, bp, penalty_eps=1.0, once=True):
+    """Direct solve of the free-slip / free-slip Stokes system with the rotation
+    penalty eps Phi G^-1 Phi^T counted once per node (once=True, the symmetric
+    operator of tn_eps_set_penalty_once) or once per node copy (the reference's
+    rule), zero-mean pressure; inputs / outputs in the duplicated layout."""
+    A = O.eps_assemble(eps_op)
+    B = O.div_assemble(mv, mp, O.FREESLIP, O.FREESLIP)
+    nu, npr = A.shape[0], B.shape[0]
+    c = sp.csr_matrix(np.ones((1, npr)))
+    ntot = nu + npr + 1
+    b = np.concatenate([O.to_unique(mv, bu, 3), O.to_unique(mp, bp, 1), [0.0]])
+    Phi = _rotation_fields_unique(mv)
+    C = penalty_eps * np.linalg.inv(Phi.T @ Phi)
+    U = Phi if once else _multiplicity_unique(mv)[:, None] * Phi
+    node_a, node_b = 3 * (mv.nr // 2), 3 * (mv.ngroups // 2 * mv.nr + mv.nr // 2)
+    edofs = [node_a, node_a + 1, node_a + 2, node_b, node_b + 1, node_b + 2]
+    d = float(A.diagonal().max())
+    Ad = A + sp.csr_matrix((np.full(6, d), (edofs, edofs)), shape=A.shape)
+    Kb = sp.bmat([[Ad, B.T, None], [B, None, c.T], [None, c, None]], format="csc")
+    lu = spl.splu(Kb, permc_spec="MMD_ATA")
+    E = np.zeros((ntot, 6))
+    E[edofs, np.arange(6)] = 1.0
+    L = np.concatenate([np.vstack([U, np.zeros((npr + 1, 3))]), E], axis=1)
+    R = np.concatenate([np.vstack([Phi, np.zeros((npr + 1, 3))]), E], axis=1)
+    Minv = np.zeros((9, 9))
+    Minv[:3, :3] = np.linalg.inv(C)
+    Minv[3:, 3:] = -np.eye(6) / d
+    y = lu.solve(b)
+    Z = lu.solve(L)
+    sol_v = y - Z @ np.linalg.solve(Minv + R.T @ Z, R.T @ y)
+    return O.from_unique(mv, sol_v[:nu], 3), O.from_unique(mp, sol_v[nu:nu + npr], 1)
