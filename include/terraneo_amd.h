@@ -48,6 +48,7 @@ typedef struct tn_mesh tn_mesh;
 typedef struct tn_eps tn_eps;
 typedef struct tn_stokes tn_stokes;
 typedef struct tn_gmg tn_gmg;
+typedef struct tn_fv tn_fv;
 
 /* Stokes solver variants (block preconditioners of block_preconditioner_2x2.hpp) */
 enum tn_stokes_method { TN_MINRES_BLOCKDIAG = 0, TN_FGMRES_BLOCKTRI = 1, TN_FGMRES_BLOCKDIAG = 2 };
@@ -345,6 +346,62 @@ int tn_pcg_solve(tn_eps* A, tn_gmg* prec, void* x, const void* b, const tn_krylo
 int tn_stokes_solve(tn_stokes* K, int method, tn_gmg* prec_u, const void* schur_diag, void* u, void* p,
                     const void* bu, const void* bp, const tn_krylov_options* o, tn_solve_info* info, double* history,
                     int history_cap, tn_stream_t s);
+
+/* ---------------------------------------------------------------- energy step (hex finite volumes)
+ * FV scalars are FP64 [10][n+1][n+1][nr+1] (r fastest) with one ghost layer,
+ * the layout of the reference's VectorFVScalar (linalg/vector_fv.hpp): real
+ * cells (xc, yc, rc) in [1, n-1]^2 x [1, nr-1] are the FE hex cells
+ * (xc-1, yc-1, rc-1) of the velocity mesh.  Velocity: the FP64 nodal vector
+ * field of that mesh, [3][10][n][n][nr].  Results repeat bit for bit (no
+ * atomics). */
+/* FV state on the velocity mesh: cell centres with their ghost layers
+ * (fv/hex/helpers.hpp:177-190 initialize_cell_centers) and the FCT work
+ * arrays (FVFCTBuffers, fct_advection_diffusion.hpp:27-76) */
+int tn_fv_create(tn_mesh* m, tn_fv** out);
+int tn_fv_destroy(tn_fv* fv);
+/* FV scalar elements including ghosts: 10 (n+1)^2 (nr+1) */
+int tn_fv_size(const tn_fv* fv, int64_t* elems);
+/* device copy of the cell centres [size][3] (ghost layers filled; the radial
+ * ghosts at the shells hold zero, as in the reference) */
+int tn_fv_cell_centers(const tn_fv* fv, double* out, tn_stream_t s);
+/* lateral ghost cells <- the neighbouring diamonds' real cells
+ * (communication/shell/fv_communication.hpp update_fv_ghost_layers); ncomp
+ * components innermost (1 for scalars, 3 for centres) */
+int tn_fv_ghost_update(tn_fv* fv, double* f, int ncomp, tn_stream_t s);
+/* largest stable explicit step min_K M_K / lambda_K (compute_dt_stable,
+ * fct_advection_diffusion.hpp:110-231); result on the host */
+int tn_fv_dt_stable(tn_fv* fv, const double* vel, double kappa, double* dt, tn_stream_t s);
+/* Dirichlet values on the shells: real boundary cells and the ghosts beyond
+ * (apply_dirichlet_bcs, fv/hex/helpers.hpp:64-106) */
+int tn_fv_apply_dirichlet(tn_fv* fv, double* f, int apply_cmb, double t_cmb, int apply_surf, double t_surf,
+                          tn_stream_t s);
+/* FCT stages (fct_predictor :404-483, fct_limiter :547-599, fct_correction
+ * :660-715): T's ghosts updated, then T_L and the antidiffusive fluxes into
+ * the state's buffers; the limiter updates T_L's ghosts, forms R+ / R- and
+ * their ghosts; the correction writes T = T_L + sum alpha f.  source: FV
+ * scalar or NULL */
+int tn_fct_predictor(tn_fv* fv, double* T, const double* vel, double dt, double kappa, const double* source,
+                     int subtract_divergence, tn_stream_t s);
+int tn_fct_limiter(tn_fv* fv, tn_stream_t s);
+int tn_fct_correction(tn_fv* fv, double* T, tn_stream_t s);
+/* device pointers of the state's T_L, fluxes [6][size], R+, R- (any may be NULL) */
+int tn_fct_buffers(tn_fv* fv, double** TL, double** antidiff, double** Rp, double** Rm);
+/* one explicit FCT step (fct_explicit_step, :719-757): predictor, Dirichlet
+ * values on T_L when apply_cmb / apply_surf, limiter, correction; T in place */
+int tn_fct_explicit_step(tn_fv* fv, double* T, const double* vel, double dt, double kappa, const double* source,
+                         int subtract_divergence, int apply_cmb, double t_cmb, int apply_surf, double t_surf,
+                         tn_stream_t s);
+/* first-order upwind step, no correction (upwind_explicit_step, :775-797) */
+int tn_fct_upwind_step(tn_fv* fv, double* T, const double* vel, double dt, double kappa, const double* source,
+                       int subtract_divergence, tn_stream_t s);
+/* right-hand side of the FV -> FE L2 projection, b_i = sum_K T_K int_K phi_i,
+ * interface summed (nodal scalar of the velocity mesh) */
+int tn_fv_to_fe_rhs(tn_fv* fv, const double* Tfv, double* b, tn_stream_t s);
+/* L2 projection FV -> FE (l2_project_fv_to_fe, fv/hex/conversion.hpp:131-211):
+ * CG on the consistent mass matrix, no preconditioner, from Tfe's content;
+ * the reference uses 1000 iterations and rtol = atol = 1e-12 */
+int tn_fv_to_fe(tn_fv* fv, const double* Tfv, double* Tfe, const tn_krylov_options* o, tn_solve_info* info,
+                double* history, int history_cap, tn_stream_t s);
 
 #ifdef __cplusplus
 }

@@ -114,6 +114,30 @@ void orc_gca_build(const orc_eps* fine, const orc_mesh* mc, double* out);
 /* variant with the geometric radial bracket (the exact Galerkin product P^T A P) */
 void orc_gca_build_geometric(const orc_eps* fine, const orc_mesh* mc, double* out);
 
+/* ---------------------------------------------------------------- FV energy step (fv_oracle.c)
+ * FV scalar [10][n+1][n+1][nr+1] with one ghost layer (real cells 1..n-1,
+ * 1..nr-1), cell centres [..][3], antidiffusive fluxes [..][6] */
+int64_t orc_fv_size(const orc_mesh* m);
+void    orc_fv_cell_centers(const orc_mesh* m, double* c);
+void    orc_fv_ghost_update(const orc_mesh* m, double* f, int ncomp);
+/* lateral ghost -> source real cell pairs (d, x, y, sd, sx, sy); returns the count */
+int     orc_fv_ghost_pairs(const orc_mesh* m, int32_t* pairs);
+void    orc_fv_geometry(const orc_mesh* m, const double* centers, const double* vel, int d, int xc, int yc, int rc,
+                        double beta[6], double* M, double S[6][3]);
+double  orc_fv_dt_stable(const orc_mesh* m, const double* centers, const double* vel, double kappa);
+void    orc_fct_predictor(const orc_mesh* m, const double* centers, const double* vel, double* T, double dt,
+                          double kappa, const double* source, int subtract_divergence, double* TL, double* antidiff);
+void    orc_fv_dirichlet(const orc_mesh* m, double* f, int apply_cmb, double t_cmb, int apply_surf, double t_surf);
+void    orc_fct_limiter(const orc_mesh* m, double* TL, const double* antidiff, double* Rp, double* Rm);
+void    orc_fct_correction(const orc_mesh* m, const double* TL, const double* antidiff, const double* Rp,
+                           const double* Rm, double* T);
+/* work: 3 FV scalars + 6 per FV cell, zero on the first call */
+void    orc_fct_explicit_step(const orc_mesh* m, const double* centers, const double* vel, double* T, double dt,
+                              double kappa, const double* source, int subtract_divergence, int apply_cmb,
+                              double t_cmb, int apply_surf, double t_surf, double* work);
+/* b = int phi_i T (nodal scalar, interface summed) */
+void    orc_fv_to_fe_rhs(const orc_mesh* m, const double* Tfv, double* b);
+
 /* ---------------------------------------------------------------- BLAS-1 */
 double orc_dot_owned(const orc_mesh* m, int ncomp, const double* a, const double* b);
 

@@ -26,5 +26,5 @@ from ._capi import (  # noqa: F401
     TNError,
     lib,
 )
-from .api import Eps, Gmg, Mesh, Stokes, pcg_solve, stokes_solve  # noqa: F401
+from .api import FV, Eps, Gmg, Mesh, Stokes, pcg_solve, stokes_solve  # noqa: F401
 from . import api  # noqa: F401

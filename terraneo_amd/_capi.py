@@ -113,6 +113,21 @@ SIGNATURES = {
     "tn_gmg_set_lambda": (i32, [vp, i32, f64]),
     "tn_pcg_solve": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, vp]),
     "tn_stokes_solve": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp]),
+    "tn_fv_create": (i32, [vp, pvp]),
+    "tn_fv_destroy": (i32, [vp]),
+    "tn_fv_size": (i32, [vp, C.POINTER(i64)]),
+    "tn_fv_cell_centers": (i32, [vp, vp, vp]),
+    "tn_fv_ghost_update": (i32, [vp, vp, i32, vp]),
+    "tn_fv_dt_stable": (i32, [vp, vp, f64, C.POINTER(f64), vp]),
+    "tn_fv_apply_dirichlet": (i32, [vp, vp, i32, f64, i32, f64, vp]),
+    "tn_fct_predictor": (i32, [vp, vp, vp, f64, f64, vp, i32, vp]),
+    "tn_fct_limiter": (i32, [vp, vp]),
+    "tn_fct_correction": (i32, [vp, vp, vp]),
+    "tn_fct_buffers": (i32, [vp, pvp, pvp, pvp, pvp]),
+    "tn_fct_explicit_step": (i32, [vp, vp, vp, f64, f64, vp, i32, i32, f64, i32, f64, vp]),
+    "tn_fct_upwind_step": (i32, [vp, vp, vp, f64, f64, vp, i32, vp]),
+    "tn_fv_to_fe_rhs": (i32, [vp, vp, vp, vp]),
+    "tn_fv_to_fe": (i32, [vp, vp, vp, vp, vp, vp, i32, vp]),
 }
 
 

@@ -369,3 +369,104 @@ def stokes_solve(K, method, prec_u, schur_diag, u, p, bu, bp, max_iterations=100
     check(lib().tn_stokes_solve(K.h, method, prec_u.h, _ptr(schur_diag), _ptr(u), _ptr(p), _ptr(bu), _ptr(bp),
                                 C.byref(o), C.byref(info), hist.ctypes.data, len(hist), _stream_ptr(stream)))
     return _info_dict(info, hist)
+
+
+# ---------------------------------------------------------------- energy step (hex finite volumes)
+class FV:
+    """FV state on a velocity mesh (cell centres, ghost map, FCT work arrays):
+    the reference's FVFCTBuffers + initialize_cell_centers.  FV scalars are
+    FP64 torch tensors of shape fv_shape = (10, n+1, n+1, nr+1) with one
+    ghost layer; the velocity is the FP64 nodal field [3, 10, n, n, nr]."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        h = C.c_void_p()
+        check(lib().tn_fv_create(mesh.h, C.byref(h)))
+        self.h = h
+        n, nr = mesh.n, mesh.nr
+        self.fv_shape = (10, n + 1, n + 1, nr + 1)
+
+    def empty(self, fill=0.0):
+        import torch
+
+        return torch.full(self.fv_shape, fill, dtype=torch.float64, device="cuda")
+
+    def cell_centers(self, stream=None):
+        import torch
+
+        c = torch.empty(self.fv_shape + (3,), dtype=torch.float64, device="cuda")
+        check(lib().tn_fv_cell_centers(self.h, _ptr(c), _stream_ptr(stream)))
+        return c
+
+    def ghost_update(self, f, ncomp=1, stream=None):
+        check(lib().tn_fv_ghost_update(self.h, _ptr(f), ncomp, _stream_ptr(stream)))
+        return f
+
+    def dt_stable(self, vel, kappa=0.0, stream=None):
+        out = C.c_double()
+        check(lib().tn_fv_dt_stable(self.h, _ptr(vel), float(kappa), C.byref(out), _stream_ptr(stream)))
+        return out.value
+
+    def apply_dirichlet(self, f, cmb=None, surf=None, stream=None):
+        check(lib().tn_fv_apply_dirichlet(self.h, _ptr(f), int(cmb is not None), float(cmb or 0.0),
+                                          int(surf is not None), float(surf or 0.0), _stream_ptr(stream)))
+        return f
+
+    def predictor(self, T, vel, dt, kappa=0.0, source=None, subtract_divergence=True, stream=None):
+        check(lib().tn_fct_predictor(self.h, _ptr(T), _ptr(vel), float(dt), float(kappa),
+                                     _ptr(source) if source is not None else None, int(bool(subtract_divergence)),
+                                     _stream_ptr(stream)))
+
+    def limiter(self, stream=None):
+        check(lib().tn_fct_limiter(self.h, _stream_ptr(stream)))
+
+    def correction(self, T, stream=None):
+        check(lib().tn_fct_correction(self.h, _ptr(T), _stream_ptr(stream)))
+        return T
+
+    def buffers(self):
+        """host copies (numpy) of the state's T_L, antidiffusive fluxes [6, ...], R+, R-"""
+        ptrs = [C.c_void_p() for _ in range(4)]
+        check(lib().tn_fct_buffers(self.h, *[C.byref(p) for p in ptrs]))
+        check(lib().tn_sync())
+        out = []
+        for p, k in zip(ptrs, (1, 6, 1, 1)):
+            a = np.empty(((k,) if k > 1 else ()) + self.fv_shape)
+            check(lib().tn_memcpy_d2h(a.ctypes.data, p, a.nbytes))
+            out.append(a)
+        return out
+
+    def explicit_step(self, T, vel, dt, kappa=0.0, source=None, subtract_divergence=True, cmb=None, surf=None,
+                      stream=None):
+        check(lib().tn_fct_explicit_step(self.h, _ptr(T), _ptr(vel), float(dt), float(kappa),
+                                         _ptr(source) if source is not None else None,
+                                         int(bool(subtract_divergence)), int(cmb is not None), float(cmb or 0.0),
+                                         int(surf is not None), float(surf or 0.0), _stream_ptr(stream)))
+        return T
+
+    def upwind_step(self, T, vel, dt, kappa=0.0, source=None, subtract_divergence=True, stream=None):
+        check(lib().tn_fct_upwind_step(self.h, _ptr(T), _ptr(vel), float(dt), float(kappa),
+                                       _ptr(source) if source is not None else None, int(bool(subtract_divergence)),
+                                       _stream_ptr(stream)))
+        return T
+
+    def to_fe_rhs(self, Tfv, b, stream=None):
+        check(lib().tn_fv_to_fe_rhs(self.h, _ptr(Tfv), _ptr(b), _stream_ptr(stream)))
+        return b
+
+    def to_fe(self, Tfv, Tfe, max_iterations=1000, rtol=1e-12, atol=1e-12, stream=None):
+        """L2 projection FV -> nodal (CG on the mass matrix from Tfe's content)"""
+        o = KrylovOptions(max_iterations, rtol, atol, 30, 0)
+        info = SolveInfo()
+        hist = np.zeros(max_iterations + 2)
+        check(lib().tn_fv_to_fe(self.h, _ptr(Tfv), _ptr(Tfe), C.byref(o), C.byref(info), hist.ctypes.data,
+                                len(hist), _stream_ptr(stream)))
+        return _info_dict(info, hist)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().tn_fv_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

@@ -5,9 +5,11 @@
 
 #include "tn_coarse.hpp"
 #include "tn_eps_march.hpp"
+#include "tn_fv.hpp"
 #include "tn_mg.hpp"
 #include "tn_objects.hpp"
 #include "tn_solvers.hpp"
+#include "tn_stokes.hpp"
 
 namespace tn {
 void set_last_error(const std::string& msg);
@@ -446,6 +448,48 @@ int tn_stokes_solve(tn_stokes* K, int method, tn_gmg* prec_u, const void* schur_
         info->restarts = restarts;
         info->prec_skipped = skipped;
     }
+    TN_API_END
+}
+
+// L2 projection of an FV scalar onto the nodal (Q1) space: M x = b with
+// b_i = sum_K T_K int_K phi_i, unpreconditioned CG on the consistent mass
+// matrix from x's content (conversion.hpp:131-211; the reference's defaults
+// are 1000 iterations, rtol = atol = 1e-12)
+int tn_fv_to_fe(tn_fv* fv, const double* Tfv, double* Tfe, const tn_krylov_options* o, tn_solve_info* info,
+                double* history, int history_cap, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(fv && Tfv && Tfe && o, "null pointer");
+    Ctx c;
+    c.s = (hipStream_t)s;
+    VecStore st;
+    const BVec X = vec1(field_of(fv->mesh, TN_F64, 1, Tfe));
+    const BVec B = st.make(X);
+    launch_fv_to_fe_rhs(fv->args(), Tfv, (double*)B.b[0].p, c.s);
+    MeshLevel& L = fv->mesh->lvl.dev();
+    launch_interface_sum<double>(L, 1, (double*)B.b[0].p, c.s);
+    struct MassOp : LinOp {
+        MeshLevel* L;
+        void apply(Ctx& c, const BVec& x, const BVec& y) override
+        {
+            double* yy = (double*)y.b[0].p;
+            launch_fill<double>(yy, L->N, 0.0, c.s);
+            MassArgs<double> a{L->d_coords.p, L->d_radii.p, L->n, L->nr, 1, (const double*)x.b[0].p, nullptr, yy};
+            launch_mass<double>(a, false, c.s);
+            launch_interface_sum<double>(*L, 1, yy, c.s);
+        }
+    } op;
+    op.L = &L;
+    std::vector<BVec> t;
+    for (int i = 0; i < 4; ++i) t.push_back(st.make(X));
+    KrylovParams kp{o->max_iterations, o->rtol, o->atol, o->restart};
+    PCG pcg(kp, t, nullptr);
+    TN_HIP(hipStreamSynchronize(c.s));
+    const auto t0 = std::chrono::steady_clock::now();
+    pcg.solve(c, op, X, B);
+    TN_HIP(hipStreamSynchronize(c.s));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fill_info(info, pcg.info, secs, history, history_cap);
     TN_API_END
 }
 

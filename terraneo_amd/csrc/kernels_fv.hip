@@ -1,0 +1,534 @@
+// Hex finite-volume energy step: flux-corrected transport (FCT) of a scalar
+// by the Stokes velocity, with two-point diffusion (FP64).
+//
+// Reference: fv/hex/operators/fct_advection_diffusion.hpp (stages),
+// fv/hex/operators/geometry_helper.hpp:178-360 (cell geometry),
+// fv/hex/helpers.hpp:64-190 (Dirichlet values, cell centres),
+// fv/hex/conversion.hpp:131-211 (FV -> FE projection right-hand side),
+// communication/shell/fv_communication.hpp (ghost layers).
+//
+// FV fields carry one ghost layer: [10][n+1][n+1][nr+1], r fastest; real
+// cells (xc, yc, rc) in [1, n-1]^2 x [1, nr-1] are the FE hex cells
+// (xc-1, yc-1, rc-1).  Every kernel runs one thread per real cell (or per
+// FE node for the projection), radially consecutive threads on radially
+// consecutive cells, so every field access of a wave is one contiguous
+// radial run.  Nothing is accumulated with atomics: every output element is
+// written by one thread, so results repeat bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "tn_fv.hpp"
+
+namespace tn {
+
+namespace {
+
+constexpr int FV_BLOCK = 256;
+
+__device__ __forceinline__ int64_t fvi(int n, int nr, int d, int x, int y, int r)
+{
+    return (((int64_t)d * (n + 1) + x) * (n + 1) + y) * (nr + 1) + r;
+}
+
+// wedge local node offsets (kernel_helpers.hpp:302-327)
+__constant__ int FV_WOFF[2][6][3] = {{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}},
+                                     {{1, 1, 0}, {0, 1, 0}, {1, 0, 0}, {1, 1, 1}, {0, 1, 1}, {1, 0, 1}}};
+
+// Felippa 3x2 wedge rule (fe/wedge/quadrature/quadrature.hpp:67-110): triangle
+// points (2/3,1/6), (1/6,2/3), (1/6,1/6) at zeta = -+1/sqrt(3), weight 1/6
+constexpr double QT_A = 0.6666666666666666, QT_B = 0.1666666666666667, QZ = 0.5773502691896257,
+                 QW = 0.1666666666666667;
+
+__device__ __forceinline__ void cell_real(int64_t t, int n, int nr, int& d, int& xc, int& yc, int& rc)
+{
+    rc = 1 + (int)(t % (nr - 1));
+    t /= (nr - 1);
+    yc = 1 + (int)(t % (n - 1));
+    t /= (n - 1);
+    xc = 1 + (int)(t % (n - 1));
+    d = (int)(t / (n - 1));
+}
+
+// lateral corner positions of the two wedges of FE hex cell (x, y)
+__device__ __forceinline__ void wedge_corners(const double* __restrict__ coords, int n, int d, int x, int y,
+                                              double P[2][3][3])
+{
+#pragma unroll
+    for (int w = 0; w < 2; ++w)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double* c = coords + 3 * (((int64_t)d * n + x + FV_WOFF[w][i][0]) * n + y + FV_WOFF[w][i][1]);
+            P[w][i][0] = c[0];
+            P[w][i][1] = c[1];
+            P[w][i][2] = c[2];
+        }
+}
+
+// |det J| of the wedge map x = r(zeta) L(xi, eta) at (xi, eta, zeta)
+// (integrands.hpp:623-670): det [r (p1-p0), r (p2-p0), (dr/2) L]
+__device__ __forceinline__ double abs_det(const double P[3][3], double r1, double r2, double xi, double eta,
+                                          double zeta)
+{
+    const double r = r1 + 0.5 * (r2 - r1) * (1 + zeta);
+    const double gr = 0.5 * (r2 - r1);
+    double J[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        J[a][0] = (P[1][a] - P[0][a]) * r;
+        J[a][1] = (P[2][a] - P[0][a]) * r;
+        J[a][2] = ((1 - xi - eta) * P[0][a] + xi * P[1][a] + eta * P[2][a]) * gr;
+    }
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    return fabs(det);
+}
+
+__device__ __forceinline__ void qpoint(int q, double& xi, double& eta, double& zeta)
+{
+    const int k = q % 3;
+    xi = k == 0 ? QT_A : QT_B;
+    eta = k == 1 ? QT_A : QT_B;
+    zeta = q < 3 ? -QZ : QZ;
+}
+
+// Geometry of real FV cell (d, xc, yc, rc): beta[f] = int_face u.n dS and
+// S[f] = int_face n dS (faces 0: x-1, 1: x+1, 2: y-1, 3: y+1, 4: r-1, 5: r+1),
+// each oriented towards the neighbour's centre, and the cell volume M.
+// Lateral faces: 2 x 2 Gauss points (lateral coordinate on [0, 1], weight
+// 1/2); radial faces: the 3-point triangle rule on both wedges.  The
+// wedge-local interpolation of u and of the node positions follows the
+// reference's shape functions.
+__device__ void cell_geometry(const FvArgs& a, const double* __restrict__ vel, int64_t N, int d, int xc, int yc,
+                              int rc, double beta[6], double& M, double S[6][3])
+{
+    const int n = a.n, nr = a.nr;
+    double P[2][3][3];
+    wedge_corners(a.coords, n, d, xc - 1, yc - 1, P);
+    const double r1 = a.radii[rc - 1], r2 = a.radii[rc];
+    const double gl = 0.57735026918962584;  // 1/sqrt(3)
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        beta[f] = 0;
+        S[f][0] = S[f][1] = S[f][2] = 0;
+    }
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        double U[6][3];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const int64_t node = (((int64_t)d * n + xc - 1 + FV_WOFF[w][i][0]) * n + yc - 1 + FV_WOFF[w][i][1]) * nr +
+                                 rc - 1 + FV_WOFF[w][i][2];
+            U[i][0] = vel[node];
+            U[i][1] = vel[N + node];
+            U[i][2] = vel[2 * N + node];
+        }
+        // faces of this wedge: its two lateral quads, then the bottom and top triangles
+        const int fq0 = w == 0 ? 0 : 1, fq1 = w == 0 ? 2 : 3;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int f = k == 0 ? fq0 : (k == 1 ? fq1 : (k == 2 ? 4 : 5));
+            const bool tri = k >= 2;
+            const int np = tri ? 3 : 4;
+            for (int q = 0; q < np; ++q) {
+                double xi, eta, zeta, wq;
+                if (tri) {
+                    xi = q == 0 ? QT_A : QT_B;
+                    eta = q == 1 ? QT_A : QT_B;
+                    zeta = f == 4 ? -1.0 : 1.0;
+                    wq = QW;
+                } else {
+                    const double s = 0.5 * ((q < 2 ? -gl : gl) + 1.0), t = (q & 1) ? gl : -gl;
+                    wq = 0.5;
+                    zeta = t;
+                    if (k == 0) {  // x-faces: xi = 0, eta = s
+                        xi = 0;
+                        eta = s;
+                    } else {  // y-faces: eta = 0, xi = s
+                        xi = s;
+                        eta = 0;
+                    }
+                }
+                // dx/dxi, dx/deta, dx/dzeta and u at the point
+                const double lat[3] = {1.0 - xi - eta, xi, eta};
+                const double rb = 0.5 * (1 - zeta), rt = 0.5 * (1 + zeta);
+                double dxi[3], deta[3], dzeta[3], u[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double xb0 = P[w][0][c] * r1, xb1 = P[w][1][c] * r1, xb2 = P[w][2][c] * r1;
+                    const double xt0 = P[w][0][c] * r2, xt1 = P[w][1][c] * r2, xt2 = P[w][2][c] * r2;
+                    dxi[c] = (xb1 - xb0) * rb + (xt1 - xt0) * rt;
+                    deta[c] = (xb2 - xb0) * rb + (xt2 - xt0) * rt;
+                    dzeta[c] = 0.5 * ((lat[0] * xt0 + lat[1] * xt1 + lat[2] * xt2) -
+                                      (lat[0] * xb0 + lat[1] * xb1 + lat[2] * xb2));
+                    u[c] = rb * (lat[0] * U[0][c] + lat[1] * U[1][c] + lat[2] * U[2][c]) +
+                           rt * (lat[0] * U[3][c] + lat[1] * U[4][c] + lat[2] * U[5][c]);
+                }
+                const double* e0 = tri ? dxi : (k == 0 ? deta : dxi);
+                const double* e1 = tri ? deta : dzeta;
+                const double nv[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2],
+                                      e0[0] * e1[1] - e0[1] * e1[0]};
+                beta[f] += wq * (u[0] * nv[0] + u[1] * nv[1] + u[2] * nv[2]);
+                S[f][0] += wq * nv[0];
+                S[f][1] += wq * nv[1];
+                S[f][2] += wq * nv[2];
+            }
+        }
+    }
+    // orientation: towards the neighbour's centre (ghost centres included)
+    const double* cc = a.centers + 3 * fvi(n, nr, d, xc, yc, rc);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const int ox = f == 0 ? -1 : (f == 1 ? 1 : 0), oy = f == 2 ? -1 : (f == 3 ? 1 : 0),
+                  orr = f == 4 ? -1 : (f == 5 ? 1 : 0);
+        const double* nc = a.centers + 3 * fvi(n, nr, d, xc + ox, yc + oy, rc + orr);
+        const double s = S[f][0] * (nc[0] - cc[0]) + S[f][1] * (nc[1] - cc[1]) + S[f][2] * (nc[2] - cc[2]);
+        if (s < 0) {
+            S[f][0] = -S[f][0];
+            S[f][1] = -S[f][1];
+            S[f][2] = -S[f][2];
+            beta[f] = -beta[f];
+        }
+    }
+    double vol = 0;
+#pragma unroll
+    for (int w = 0; w < 2; ++w)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            double xi, eta, zeta;
+            qpoint(q, xi, eta, zeta);
+            vol += abs_det(P[w], r1, r2, xi, eta, zeta) * QW;
+        }
+    M = vol;
+}
+
+__device__ __forceinline__ int64_t neighbour(int n, int nr, int d, int xc, int yc, int rc, int f)
+{
+    const int ox = f == 0 ? -1 : (f == 1 ? 1 : 0), oy = f == 2 ? -1 : (f == 3 ? 1 : 0),
+              orr = f == 4 ? -1 : (f == 5 ? 1 : 0);
+    return fvi(n, nr, d, xc + ox, yc + oy, rc + orr);
+}
+
+// ---------------------------------------------------------------- kernels
+
+// cell centre = (int_K x) / |K| with the Felippa rule on both wedges (helpers.hpp:123-175)
+__global__ __launch_bounds__(FV_BLOCK) void k_fv_centers(FvArgs a, double* __restrict__ c)
+{
+    const int n = a.n, nr = a.nr;
+    const int64_t cells = 10LL * (n - 1) * (n - 1) * (nr - 1);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells) return;
+    int d, xc, yc, rc;
+    cell_real(t, n, nr, d, xc, yc, rc);
+    double P[2][3][3];
+    wedge_corners(a.coords, n, d, xc - 1, yc - 1, P);
+    const double r1 = a.radii[rc - 1], r2 = a.radii[rc];
+    double vol = 0, I[3] = {0, 0, 0};
+#pragma unroll
+    for (int w = 0; w < 2; ++w)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            double xi, eta, zeta;
+            qpoint(q, xi, eta, zeta);
+            const double Jw = abs_det(P[w], r1, r2, xi, eta, zeta) * QW;
+            vol += Jw;
+            const double rr = r1 + 0.5 * (r2 - r1) * (1 + zeta);
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                I[k] += rr * ((1 - xi - eta) * P[w][0][k] + xi * P[w][1][k] + eta * P[w][2][k]) * Jw;
+        }
+    double* o = c + 3 * fvi(n, nr, d, xc, yc, rc);
+    o[0] = I[0] / vol;
+    o[1] = I[1] / vol;
+    o[2] = I[2] / vol;
+}
+
+// lateral ghost cells <- the neighbouring diamonds' real cells (real radial range)
+__global__ __launch_bounds__(FV_BLOCK) void k_fv_ghost(const int32_t* __restrict__ pairs, int cnt, int nr,
+                                                       int ncomp, double* __restrict__ f)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)cnt * (nr - 1)) return;
+    const int rc = 1 + (int)(t % (nr - 1));
+    const int k = (int)(t / (nr - 1));
+    const int64_t dst = ((int64_t)pairs[2 * k] * (nr + 1) + rc) * ncomp;
+    const int64_t src = ((int64_t)pairs[2 * k + 1] * (nr + 1) + rc) * ncomp;
+    for (int c = 0; c < ncomp; ++c) f[dst + c] = f[src + c];
+}
+
+// local stable time step M / lambda per cell, minimum per block
+// (fct_advection_diffusion.hpp:110-164)
+__global__ __launch_bounds__(FV_BLOCK) void k_fv_dt(FvArgs a, const double* __restrict__ vel, double kappa,
+                                                    double* __restrict__ part)
+{
+    const int n = a.n, nr = a.nr;
+    const int64_t cells = 10LL * (n - 1) * (n - 1) * (nr - 1);
+    const int64_t N = 10LL * n * n * nr;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double dtc = 1e300;
+    if (t < cells) {
+        int d, xc, yc, rc;
+        cell_real(t, n, nr, d, xc, yc, rc);
+        double beta[6], M, S[6][3];
+        cell_geometry(a, vel, N, d, xc, yc, rc, beta, M, S);
+        const double* ci = a.centers + 3 * fvi(n, nr, d, xc, yc, rc);
+        double lambda = 0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            if (beta[f] < 0) lambda -= beta[f];
+            if (kappa > 0) {
+                const double* cj = a.centers + 3 * neighbour(n, nr, d, xc, yc, rc, f);
+                const double den = S[f][0] * (cj[0] - ci[0]) + S[f][1] * (cj[1] - ci[1]) + S[f][2] * (cj[2] - ci[2]);
+                if (den > 0) lambda += kappa * (S[f][0] * S[f][0] + S[f][1] * S[f][1] + S[f][2] * S[f][2]) / den;
+            }
+        }
+        dtc = lambda > 0 ? M / lambda : 1e30;
+    }
+    // block minimum (order-independent)
+    __shared__ double red[FV_BLOCK / 64];
+    for (int o = 32; o > 0; o >>= 1) dtc = fmin(dtc, __shfl_xor(dtc, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dtc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int k = 1; k < FV_BLOCK / 64; ++k) m = fmin(m, red[k]);
+        part[blockIdx.x] = m;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_min_finalize(const double* __restrict__ part, int nblk,
+                                                      double* __restrict__ out)
+{
+    double m = 1e300;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) m = fmin(m, part[b]);
+    for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o));
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+}
+
+// predictor (fct_advection_diffusion.hpp:261-402): low-order upwind +
+// two-point diffusion T_L, and the pre-scaled antidiffusive fluxes
+// (dt / M) (|beta| / 2) (T_i - T_j), stored face-major [6][fv cells]
+__global__ __launch_bounds__(FV_BLOCK) void k_fct_predictor(FvArgs a, const double* __restrict__ vel,
+                                                            const double* __restrict__ T, double dt, double kappa,
+                                                            const double* __restrict__ source, int subtract_div,
+                                                            double* __restrict__ TL, double* __restrict__ ad,
+                                                            int64_t fvsize)
+{
+    const int n = a.n, nr = a.nr;
+    const int64_t cells = 10LL * (n - 1) * (n - 1) * (nr - 1);
+    const int64_t N = 10LL * n * n * nr;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells) return;
+    int d, xc, yc, rc;
+    cell_real(t, n, nr, d, xc, yc, rc);
+    double beta[6], M, S[6][3];
+    cell_geometry(a, vel, N, d, xc, yc, rc, beta, M, S);
+    const int64_t i = fvi(n, nr, d, xc, yc, rc);
+    const double Ti = T[i];
+    const double* ci = a.centers + 3 * i;
+    const double sc = dt / M;
+    double lo_diag = 0, lo_update = 0, bsum = 0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const int64_t j = neighbour(n, nr, d, xc, yc, rc, f);
+        const double Tj = T[j];
+        if (beta[f] >= 0)
+            lo_diag += beta[f];
+        else
+            lo_update += beta[f] * Tj;
+        bsum += beta[f];
+        const double* cj = a.centers + 3 * j;
+        const double den = S[f][0] * (cj[0] - ci[0]) + S[f][1] * (cj[1] - ci[1]) + S[f][2] * (cj[2] - ci[2]);
+        const double dc = kappa * ((S[f][0] * S[f][0] + S[f][1] * S[f][1] + S[f][2] * S[f][2]) / den);
+        lo_diag += dc;
+        lo_update -= dc * Tj;
+        ad[f * fvsize + i] = sc * (fabs(beta[f]) / 2.0) * (Ti - Tj);
+    }
+    if (subtract_div) lo_diag -= bsum;
+    double v = Ti - sc * (lo_diag * Ti + lo_update);
+    if (source) v += dt * source[i];
+    TL[i] = v;
+}
+
+// Dirichlet values on the shells: the real boundary cell and the ghost beyond (helpers.hpp:64-96)
+__global__ __launch_bounds__(FV_BLOCK) void k_fv_dirichlet(int n, int nr, double* __restrict__ f, int apply_cmb,
+                                                           double t_cmb, int apply_surf, double t_surf)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 10LL * (n - 1) * (n - 1)) return;
+    const int yc = 1 + (int)(t % (n - 1));
+    const int xc = 1 + (int)((t / (n - 1)) % (n - 1));
+    const int d = (int)(t / ((int64_t)(n - 1) * (n - 1)));
+    if (apply_cmb) {
+        f[fvi(n, nr, d, xc, yc, 1)] = t_cmb;
+        f[fvi(n, nr, d, xc, yc, 0)] = t_cmb;
+    }
+    if (apply_surf) {
+        f[fvi(n, nr, d, xc, yc, nr - 1)] = t_surf;
+        f[fvi(n, nr, d, xc, yc, nr)] = t_surf;
+    }
+}
+
+// limiter (fct_advection_diffusion.hpp:485-537): Zalesak R+ / R-
+__global__ __launch_bounds__(FV_BLOCK) void k_fct_limiter(int n, int nr, const double* __restrict__ TL,
+                                                          const double* __restrict__ ad, int64_t fvsize,
+                                                          double* __restrict__ Rp, double* __restrict__ Rm)
+{
+    const int64_t cells = 10LL * (n - 1) * (n - 1) * (nr - 1);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells) return;
+    int d, xc, yc, rc;
+    cell_real(t, n, nr, d, xc, yc, rc);
+    const int64_t i = fvi(n, nr, d, xc, yc, rc);
+    const double ti = TL[i];
+    double pp = 0, pm = 0, tmax = ti, tmin = ti;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const double fij = ad[f * fvsize + i];
+        if (fij > 0)
+            pp += fij;
+        else
+            pm += fij;
+        const double tj = TL[neighbour(n, nr, d, xc, yc, rc, f)];
+        tmax = fmax(tmax, tj);
+        tmin = fmin(tmin, tj);
+    }
+    Rp[i] = pp > 0 ? fmin(1.0, (tmax - ti) / pp) : 1.0;
+    Rm[i] = pm < 0 ? fmin(1.0, (tmin - ti) / pm) : 1.0;
+}
+
+// correction (fct_advection_diffusion.hpp:604-658): T = T_L + sum_j alpha_ij f_ij
+__global__ __launch_bounds__(FV_BLOCK) void k_fct_correction(int n, int nr, const double* __restrict__ TL,
+                                                             const double* __restrict__ ad, int64_t fvsize,
+                                                             const double* __restrict__ Rp,
+                                                             const double* __restrict__ Rm, double* __restrict__ T)
+{
+    const int64_t cells = 10LL * (n - 1) * (n - 1) * (nr - 1);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells) return;
+    int d, xc, yc, rc;
+    cell_real(t, n, nr, d, xc, yc, rc);
+    const int64_t i = fvi(n, nr, d, xc, yc, rc);
+    const double rpi = Rp[i], rmi = Rm[i];
+    double corr = 0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const int64_t j = neighbour(n, nr, d, xc, yc, rc, f);
+        const double fij = ad[f * fvsize + i];
+        const double al = fij > 0 ? fmin(rpi, Rm[j]) : fmin(rmi, Rp[j]);
+        corr += al * fij;
+    }
+    T[i] = TL[i] + corr;
+}
+
+// right-hand side of the FV -> FE L2 projection (conversion.hpp:151-200):
+// b_i = sum over the cells K around node i of T_K int_K phi_i, gathered per
+// node copy (no atomics); the interface sum follows
+__global__ __launch_bounds__(FV_BLOCK) void k_fv_to_fe_rhs(FvArgs a, const double* __restrict__ Tfv,
+                                                           double* __restrict__ b)
+{
+    const int n = a.n, nr = a.nr;
+    const int64_t N = 10LL * n * n * nr;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int r = (int)(t % nr);
+    const int y = (int)((t / nr) % n);
+    const int x = (int)((t / ((int64_t)nr * n)) % n);
+    const int d = (int)(t / ((int64_t)nr * n * n));
+    double acc = 0;
+    for (int cx = x - 1; cx <= x; ++cx) {
+        if (cx < 0 || cx > n - 2) continue;
+        for (int cy = y - 1; cy <= y; ++cy) {
+            if (cy < 0 || cy > n - 2) continue;
+            double P[2][3][3];
+            wedge_corners(a.coords, n, d, cx, cy, P);
+            for (int cr = r - 1; cr <= r; ++cr) {
+                if (cr < 0 || cr > nr - 2) continue;
+                const double r1 = a.radii[cr], r2 = a.radii[cr + 1];
+                const double tv = Tfv[fvi(n, nr, d, cx + 1, cy + 1, cr + 1)];
+                for (int w = 0; w < 2; ++w) {
+                    int li = -1;
+#pragma unroll
+                    for (int i = 0; i < 6; ++i)
+                        if (FV_WOFF[w][i][0] == x - cx && FV_WOFF[w][i][1] == y - cy && FV_WOFF[w][i][2] == r - cr)
+                            li = i;
+                    if (li < 0) continue;
+                    double s = 0;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        double xi, eta, zeta;
+                        qpoint(q, xi, eta, zeta);
+                        const double lat[3] = {1.0 - xi - eta, xi, eta};
+                        const double sh = lat[li % 3] * (li < 3 ? 0.5 * (1 - zeta) : 0.5 * (1 + zeta));
+                        s += sh * abs_det(P[w], r1, r2, xi, eta, zeta) * QW;
+                    }
+                    acc += s * tv;
+                }
+            }
+        }
+    }
+    b[t] = acc;
+}
+
+int blocks_for(int64_t work) { return (int)((work + FV_BLOCK - 1) / FV_BLOCK); }
+
+}  // namespace
+
+int64_t fv_real_cells(int n, int nr) { return 10LL * (n - 1) * (n - 1) * (nr - 1); }
+int64_t fv_size(int n, int nr) { return 10LL * (n + 1) * (n + 1) * (nr + 1); }
+int fv_dt_blocks(int n, int nr) { return blocks_for(fv_real_cells(n, nr)); }
+
+void launch_fv_centers(const FvArgs& a, double* c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fv_centers, dim3(blocks_for(fv_real_cells(a.n, a.nr))), dim3(FV_BLOCK), 0, s, a, c);
+}
+
+void launch_fv_ghost(const int32_t* pairs, int cnt, int nr, int ncomp, double* f, hipStream_t s)
+{
+    if (cnt == 0) return;
+    hipLaunchKernelGGL(k_fv_ghost, dim3(blocks_for((int64_t)cnt * (nr - 1))), dim3(FV_BLOCK), 0, s, pairs, cnt, nr,
+                       ncomp, f);
+}
+
+void launch_fv_dt(const FvArgs& a, const double* vel, double kappa, double* part, double* out, hipStream_t s)
+{
+    const int nb = fv_dt_blocks(a.n, a.nr);
+    hipLaunchKernelGGL(k_fv_dt, dim3(nb), dim3(FV_BLOCK), 0, s, a, vel, kappa, part);
+    hipLaunchKernelGGL(k_min_finalize, dim3(1), dim3(256), 0, s, part, nb, out);
+}
+
+void launch_fct_predictor(const FvArgs& a, const double* vel, const double* T, double dt, double kappa,
+                          const double* source, int subtract_div, double* TL, double* ad, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fct_predictor, dim3(blocks_for(fv_real_cells(a.n, a.nr))), dim3(FV_BLOCK), 0, s, a, vel, T,
+                       dt, kappa, source, subtract_div, TL, ad, fv_size(a.n, a.nr));
+}
+
+void launch_fv_dirichlet(int n, int nr, double* f, int apply_cmb, double t_cmb, int apply_surf, double t_surf,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fv_dirichlet, dim3(blocks_for(10LL * (n - 1) * (n - 1))), dim3(FV_BLOCK), 0, s, n, nr, f,
+                       apply_cmb, t_cmb, apply_surf, t_surf);
+}
+
+void launch_fct_limiter(int n, int nr, const double* TL, const double* ad, double* Rp, double* Rm, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fct_limiter, dim3(blocks_for(fv_real_cells(n, nr))), dim3(FV_BLOCK), 0, s, n, nr, TL, ad,
+                       fv_size(n, nr), Rp, Rm);
+}
+
+void launch_fct_correction(int n, int nr, const double* TL, const double* ad, const double* Rp, const double* Rm,
+                           double* T, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fct_correction, dim3(blocks_for(fv_real_cells(n, nr))), dim3(FV_BLOCK), 0, s, n, nr, TL,
+                       ad, fv_size(n, nr), Rp, Rm, T);
+}
+
+void launch_fv_to_fe_rhs(const FvArgs& a, const double* Tfv, double* b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fv_to_fe_rhs, dim3(blocks_for(10LL * a.n * a.n * a.nr)), dim3(FV_BLOCK), 0, s, a, Tfv, b);
+}
+
+}  // namespace tn
