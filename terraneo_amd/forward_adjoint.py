@@ -1,0 +1,246 @@
+"""Forward + adjoint free-slip Stokes run with the log-viscosity gradient, at
+full size on one GPU.
+
+Default workload: velocity lateral level 8 (257 x 257 per diamond) x 129
+radial nodes (255 M velocity DoF), free-slip on both shells, per-cell
+log-viscosity from random harmonics (seed 6) normalised to [0, ln 1e6] plus
+ln 100 in the cells above r = 0.95, buoyancy forcing (seed 2).  The velocity
+multigrid (6 levels, Chebyshev degree 3, Galerkin coarse matrices) runs in
+FP32 with an FP64 dense coarsest solve, inside FP64 FGMRES(30) with the
+block-triangular preconditioner, to rel. res 1e-8 for both solves.  The
+rotation penalty is counted once per node, so K = K^T and the adjoint system
+is K itself.
+
+  forward   K (u, p) = b
+  objective J = 1/2 sum over owned outer-shell nodes |u|^2 (u_obs = 0)
+  adjoint   K (l, q) = (dJ/du, 0)   (dJ/du summed over node copies, free-slip projected)
+  gradient  dJ/dm[cell] = -eta_cell (T l)^T A_cell(1) u   (tn_eps_grad_log_cell_eta)
+
+Each solve runs as warm-started calls of whole FGMRES(30) cycles (a restart
+recomputes r = b - K x, so a new call continues the same iteration), with the
+stopping test fixed to 1e-8 of the first call's initial residual; every call
+prints one line.  Also measured: the FP64 and FP32 apply at this size (HIP
+events on the launch stream) and the gradient kernel against a central
+difference of l^T A(m) u along one smooth direction (two applies, no solve).
+
+    python -m terraneo_amd.forward_adjoint [--lat 8 --rad 7 --levels 6] [--json out.json]
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def chunked_solve(pr, bu, bp, rtol, max_iterations, chunk, label):
+    """FGMRES(30) solve from zero in calls of `chunk` iterations (a multiple of
+    the restart length), stopping at rtol times the first residual."""
+    import torch
+
+    u = torch.zeros(3 * pr.vel.N, **pr.f64)
+    p = torch.zeros(pr.pre.N, **pr.f64)
+    total, seconds, r0, res, calls = 0, 0.0, None, None, 0
+    while total < max_iterations:
+        n = min(chunk, max_iterations - total)
+        kw = dict(rtol=rtol, atol=1e-300) if r0 is None else dict(rtol=1e-300, atol=rtol * r0)
+        u, p, info = pr.solve(bu, bp, max_iterations=n, restart=30, u=u, p=p, pressure_mean_free=True, **kw)
+        calls += 1
+        if r0 is None:
+            r0 = info["initial_residual"]
+        total += info["iterations"]
+        seconds += info["seconds"]
+        res = info["final_residual"] / r0
+        _log(f"[{label}] {total} iterations, rel. res {res:.3e}, {seconds:.1f} s")
+        if res <= rtol or info["iterations"] < n:
+            break
+    return u, p, dict(iterations=total, seconds=seconds, rel_residual=res, calls=calls)
+
+
+def time_apply(op, x, y, reps=10):
+    """mean ms of op.apply over reps launches (HIP events on the current stream)"""
+    import torch
+
+    for _ in range(2):
+        op.apply(x, y)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        op.apply(x, y)
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def build_problem(lat=8, rad=7, levels=6, cheby_iterations=2, rtol=1e-8, prec_dtype="f32"):
+    """The module's workload set up on cuda:0: (problem, m, bu, bp, coords, radii),
+    m the per-cell log-viscosity (torch FP64 on the device); prec_dtype "f64"
+    runs the V-cycle in FP64 instead."""
+    import numpy as np
+    import torch
+
+    from . import api
+    from . import workloads as W
+    from ._capi import TN_FREESLIP
+    from .problem import StokesProblem
+
+    dev = torch.device("cuda:0")
+    pr = StokesProblem(lat, rad, levels, bc=(TN_FREESLIP, TN_FREESLIP), device=dev, prec_dtype=prec_dtype,
+                       penalty="once",
+                       gmg=api.gmg_options(cheby_order=3, cheby_iterations=cheby_iterations, power_iterations=10,
+                                           coarse_solver="dense"))
+    coords, radii = pr.coords()
+    m = W.cell_log_viscosity(coords, radii, 6, 1e6)
+    rmid = 0.5 * (radii[:-1] + radii[1:])
+    m = m + math.log(100.0) * (rmid > 0.95)[None, None, None, :]
+    mt = torch.from_numpy(np.ascontiguousarray(m)).to(dev)
+    f = torch.from_numpy(W.buoyancy(coords, radii)).to(dev)
+    pr.set_viscosity(eta_cell=torch.exp(mt).contiguous())
+    bu, bp = pr.rhs_from_force(f)
+    del f
+    pr.solve(bu, bp, rtol=rtol, max_iterations=1, pressure_mean_free=True)  # eigenvalue estimates
+    torch.cuda.synchronize()
+    return pr, mt, bu, bp, coords, radii
+
+
+def surface_objective(pr, u):
+    """J = 1/2 sum over owned outer-shell nodes |u|^2 and its adjoint right-hand
+    side (dJ/du summed over the node copies, free-slip projected)."""
+    import torch
+
+    from . import api
+
+    vel = pr.vel
+    own = torch.from_numpy(vel.owned_mask().reshape(10, vel.n, vel.n, vel.nr).astype(bool)).to(u.device)
+    surf = torch.zeros_like(own)
+    surf[..., -1] = True
+    mask = (own & surf).reshape(-1).repeat(3).to(torch.float64)
+    J = float(0.5 * (mask * u * u).sum())
+    g = (mask * u).contiguous()
+    vel.interface_sum(g, 3)
+    for which in (1, 2):
+        api.freeslip_project(vel, g, which)
+    return J, g
+
+
+def bilinear_derivative(pr, m, u, lam, d, h=1e-4):
+    """central difference of l^T A(m) u along d (two applies at m +- h d)"""
+    import torch
+
+    from . import api
+
+    vel = pr.vel
+    y = torch.empty_like(u)
+    vals = []
+    for sgn in (1.0, -1.0):
+        op = api.Eps(vel, None, pr.bc[0], pr.bc[1], eta_cell=torch.exp(m + sgn * h * d).contiguous())
+        op.set_penalty_once(pr.penalty == "once")
+        op.apply(u, y)
+        vals.append(vel.dot(lam, y, 3))
+        del op
+    return (vals[0] - vals[1]) / (2 * h)
+
+
+def run(lat=8, rad=7, levels=6, cheby_iterations=2, max_iterations=6000, chunk=240, rtol=1e-8):
+    import numpy as np
+    import torch
+
+    from . import workloads as W
+
+    t0 = time.perf_counter()
+    pr, mt, bu, bp, coords, radii = build_problem(lat, rad, levels, cheby_iterations, rtol)
+    vel = pr.vel
+    setup = time.perf_counter() - t0
+    free, total = torch.cuda.mem_get_info()
+    used_setup = (total - free) / 2**30
+    _log(f"setup {setup:.1f} s: velocity {3 * vel.N} DoF, pressure {pr.pre.N} DoF, "
+         f"device memory in use {used_setup:.1f} GiB")
+
+    u, p, fwd = chunked_solve(pr, bu, bp, rtol, max_iterations, chunk, "forward")
+    fwd["true_rel_residual"] = pr.residual(u, p, bu, bp, pressure_mean_free=True)
+    _log(f"forward: true rel. res {fwd['true_rel_residual']:.3e}")
+
+    J, g = surface_objective(pr, u)
+    gp = torch.zeros(pr.pre.N, **pr.f64)
+    lam, q, adj = chunked_solve(pr, g, gp, rtol, max_iterations, chunk, "adjoint")
+    adj["true_rel_residual"] = pr.residual(lam, q, g, gp, pressure_mean_free=True)
+    _log(f"adjoint: true rel. res {adj['true_rel_residual']:.3e}")
+    del q, gp
+
+    grad = torch.empty_like(mt)
+    pr.A.grad_log_cell_eta(u, lam, grad)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    pr.A.grad_log_cell_eta(u, lam, grad)
+    b.record()
+    torch.cuda.synchronize()
+    grad_ms = a.elapsed_time(b)
+
+    # gradient kernel against a central difference of l^T A(m) u along one direction
+    d = torch.from_numpy(np.ascontiguousarray(W.cell_smooth_normal(coords, radii, 1000))).to(mt.device)
+    fd = bilinear_derivative(pr, mt, u, lam, d)
+    an = -float((grad * d).sum())
+    grad_fd_rel = abs(an - fd) / abs(fd)
+    _log(f"gradient kernel {grad_ms:.3f} ms; d/dm (l^T A u) along one direction: kernel {an:.10e} "
+         f"central difference {fd:.10e} rel {grad_fd_rel:.2e}")
+
+    # apply at this size: FP64 operator, and FP32 (the V-cycle's fine level)
+    n_nodes = vel.N
+    y = torch.empty_like(u)
+    ms64 = time_apply(pr.A, u, y)
+    u32 = u.float()
+    y32 = torch.empty_like(u32)
+    ms32 = time_apply(pr.ops[-1], u32, y32)
+    del u32, y32
+    _log(f"apply FP64 {ms64:.3f} ms ({3 * n_nodes / ms64 / 1e6:.1f} GDoF/s, "
+         f"{56 * n_nodes / ms64 / 1e9:.3f} TB/s algorithmic), "
+         f"FP32 {ms32:.3f} ms ({3 * n_nodes / ms32 / 1e6:.1f} GDoF/s)")
+
+    n = 2 ** lat + 1
+    nr = 2 ** rad + 1
+    return {
+        "workload": f"free-slip Stokes forward + adjoint, velocity lateral level {lat} ({n}x{n} per diamond) x "
+                    f"{nr} radial, {levels}-level FP32 GMG (Chebyshev degree 3 x{cheby_iterations}, FP32 Galerkin "
+                    f"matrices, FP64 dense coarsest) in FP64 FGMRES(30), rel. res {rtol:g}, per-cell viscosity "
+                    "contrast 1e6 (seed 6) x100 for r > 0.95, penalty once per node",
+        "velocity_dof": 3 * n_nodes,
+        "setup_seconds": round(setup, 2),
+        "forward": fwd,
+        "adjoint": adj,
+        "objective": J,
+        "gradient_kernel_ms": round(grad_ms, 4),
+        "gradient_vs_central_difference_of_bilinear_form": float(f"{grad_fd_rel:.3e}"),
+        "apply_ms": {"f64": round(ms64, 4), "f32": round(ms32, 4)},
+        "apply_gdof_per_s": {"f64": round(3 * n_nodes / ms64 / 1e6, 2), "f32": round(3 * n_nodes / ms32 / 1e6, 2)},
+        "apply_hbm_fraction_f64": round(56 * n_nodes / ms64 / 1e9 / 8.0, 4),
+        "device_memory_after_setup_gib": round(used_setup, 1),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--lat", type=int, default=8)
+    ap.add_argument("--rad", type=int, default=7)
+    ap.add_argument("--levels", type=int, default=6)
+    ap.add_argument("--cheby-iterations", type=int, default=2)
+    ap.add_argument("--max-iterations", type=int, default=6000)
+    ap.add_argument("--chunk", type=int, default=240)
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    if a.chunk % 30:
+        sys.exit("--chunk must be a multiple of the restart length 30")
+    out = run(a.lat, a.rad, a.levels, a.cheby_iterations, a.max_iterations, a.chunk)
+    line = json.dumps(out)
+    print(line, flush=True)
+    if a.json:
+        with open(a.json, "w") as fh:
+            fh.write(line + "\n")
+
+
+if __name__ == "__main__":
+    main()
