@@ -49,6 +49,7 @@ typedef struct tn_eps tn_eps;
 typedef struct tn_stokes tn_stokes;
 typedef struct tn_gmg tn_gmg;
 typedef struct tn_fv tn_fv;
+typedef struct tn_xdmf tn_xdmf;
 
 /* Stokes solver variants (block preconditioners of block_preconditioner_2x2.hpp) */
 enum tn_stokes_method { TN_MINRES_BLOCKDIAG = 0, TN_FGMRES_BLOCKTRI = 1, TN_FGMRES_BLOCKDIAG = 2 };
@@ -402,6 +403,26 @@ int tn_fv_to_fe_rhs(tn_fv* fv, const double* Tfv, double* b, tn_stream_t s);
  * the reference uses 1000 iterations and rtol = atol = 1e-12 */
 int tn_fv_to_fe(tn_fv* fv, const double* Tfv, double* Tfe, const tn_krylov_options* o, tn_solve_info* info,
                 double* history, int history_cap, tn_stream_t s);
+
+/* ---------------------------------------------------------------- XDMF output and checkpoints
+ * The reference's directory format (io/xdmf.hpp:12-146) for one subdomain per
+ * diamond: geometry.bin, topology.bin, checkpoint_metadata.bin (version 1),
+ * <name>_<step>.bin and step_<step>.xmf.  XDMFOutput (:147-1090): create,
+ * add device fields (this library's layout, dtype TN_F32 / TN_F64, vec_dim 1
+ * or 3, written as out_bytes 4 / 8 per value), then write() once per step
+ * (copies the fields to the host; the first call also writes the mesh,
+ * topology and metadata).  points_bytes / connectivity_bytes: 4 or 8 (the
+ * reference's defaults 4 / 8). */
+int tn_xdmf_create(const tn_mesh* m, const char* directory, int points_bytes, int connectivity_bytes,
+                   tn_xdmf** out);
+int tn_xdmf_destroy(tn_xdmf* x);
+int tn_xdmf_add(tn_xdmf* x, const char* name, int dtype, int vec_dim, int out_bytes, const void* field);
+int tn_xdmf_set_write_counter(tn_xdmf* x, int step);
+int tn_xdmf_write(tn_xdmf* x, tn_stream_t s);
+/* read_xdmf_checkpoint_grid (:1282-1420): field `name` of `step` into a device
+ * array of this library's layout */
+int tn_xdmf_read_checkpoint_grid(const tn_mesh* m, const char* directory, const char* name, int step, int dtype,
+                                 int vec_dim, void* field, tn_stream_t s);
 
 #ifdef __cplusplus
 }

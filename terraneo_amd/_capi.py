@@ -128,6 +128,12 @@ SIGNATURES = {
     "tn_fct_upwind_step": (i32, [vp, vp, vp, f64, f64, vp, i32, vp]),
     "tn_fv_to_fe_rhs": (i32, [vp, vp, vp, vp]),
     "tn_fv_to_fe": (i32, [vp, vp, vp, vp, vp, vp, i32, vp]),
+    "tn_xdmf_create": (i32, [vp, C.c_char_p, i32, i32, pvp]),
+    "tn_xdmf_destroy": (i32, [vp]),
+    "tn_xdmf_add": (i32, [vp, C.c_char_p, i32, i32, i32, vp]),
+    "tn_xdmf_set_write_counter": (i32, [vp, i32]),
+    "tn_xdmf_write": (i32, [vp, vp]),
+    "tn_xdmf_read_checkpoint_grid": (i32, [vp, C.c_char_p, C.c_char_p, i32, i32, i32, vp, vp]),
 }
 
 

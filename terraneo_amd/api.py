@@ -470,3 +470,45 @@ class FV:
                 self.h = None
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- XDMF output (C ABI writer)
+class XDMFWriter:
+    """tn_xdmf_*: the reference's XDMF / checkpoint writer over device fields
+    (torch tensors in this library's layout, kept alive while registered)."""
+
+    def __init__(self, mesh, directory, points_bytes=4, connectivity_bytes=8):
+        self.mesh = mesh
+        self._keep = []
+        h = C.c_void_p()
+        check(lib().tn_xdmf_create(mesh.h, str(directory).encode(), points_bytes, connectivity_bytes, C.byref(h)))
+        self.h = h
+
+    def add(self, name, field, vec_dim=1, out_bytes=4):
+        self._keep.append(field)
+        check(lib().tn_xdmf_add(self.h, name.encode(), dtype_code(field.dtype), vec_dim, out_bytes, _ptr(field)))
+
+    def set_write_counter(self, step):
+        check(lib().tn_xdmf_set_write_counter(self.h, int(step)))
+
+    def write(self, stream=None):
+        check(lib().tn_xdmf_write(self.h, _stream_ptr(stream)))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().tn_xdmf_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def read_checkpoint_grid(mesh, directory, name, step, vec_dim=1, dtype=None, stream=None):
+    """tn_xdmf_read_checkpoint_grid: a checkpointed field as a device tensor"""
+    import torch
+
+    dtype = dtype or torch.float64
+    out = torch.empty(vec_dim * mesh.N, dtype=dtype, device="cuda")
+    check(lib().tn_xdmf_read_checkpoint_grid(mesh.h, str(directory).encode(), name.encode(), int(step),
+                                             dtype_code(dtype), vec_dim, _ptr(out), _stream_ptr(stream)))
+    return out

@@ -15,15 +15,20 @@ Per time step, as the reference app does:
 4. one row per step in a timer table (the reference prints its FGMRES table
    and a util::Timer per step).
 
-The energy step that follows in the reference (FV/FCT advection-diffusion of T)
-is not part of this path: the caller supplies T for every step.
+The energy step that follows (FV/FCT advection-diffusion of T, :736-1012) is
+`MantleCirculationEnergy`: FV temperature at the cell centres with Dirichlet
+shells (T_cmb = 1, T_surface = 0), the stable step dt_scaling x dt_stable from
+the new velocity, `energy_substeps` explicit FCT steps (each followed by the
+Dirichlet values), and the L2 projection of T to the nodes, which the next
+Stokes right-hand side uses.  `--energy` couples the two halves; without it
+the caller supplies T for every step.
 
 Defaults are the app's (apps/mantlecirculation/src/parameters.hpp:14-67): mesh
 levels 1..4, r in [0.5, 1], no-slip shells, Ra = 1e5, FGMRES(10) with at most 10
 iterations per step to rel. 1e-6 / abs. 1e-12, one V-cycle with Chebyshev order
 2 and 2 smoothing steps, 10 power iterations, coarsest-level PCG(50, 1e-6, 1e-16).
 
-    python -m terraneo_amd.app [--lat 4] [--steps 5] [--free-slip] [--prec f32] [--xdmf DIR]
+    python -m terraneo_amd.app [--lat 4] [--steps 5] [--free-slip] [--prec f32] [--energy] [--xdmf DIR]
 
 With --xdmf, every step also writes the velocity and the temperature as an XDMF
 time series / checkpoint (terraneo_amd.xdmf), as the app's output block does.
@@ -126,6 +131,65 @@ class MantleCirculationStokes:
         return "\n".join(lines)
 
 
+class MantleCirculationEnergy:
+    """The FCT energy step of the app's time loop on the velocity mesh of
+    `stokes` (mantlecirculation.cpp:736-775 set-up, :935-1012 per step).
+
+    Initial temperature: ((r_max - |c|) / (r_max - r_min))^5 at the cell
+    centres c (FVInitialConditionInterpolator, :198-216) plus uniform noise in
+    [-0.1, 0.1], clamped to [0, 1] (FVNoiseAdder, :218-234; drawn from torch's
+    generator with the app's seed 12345, so the values differ from Kokkos'
+    random pool), Dirichlet values on the shells, ghost layers, and the L2
+    projection to the nodes."""
+
+    def __init__(self, stokes, diffusivity=1.0, dt_scaling=0.5, substeps=1, t_cmb=1.0, t_surface=0.0,
+                 internal_heating=None, noise=0.1, seed=12345):
+        torch = _torch()
+        self.stokes = stokes
+        prm = stokes.prm
+        vel = stokes.pr.vel
+        self.fv = api.FV(vel)
+        self.kappa, self.dt_scaling, self.substeps = diffusivity, dt_scaling, substeps
+        self.t_cmb, self.t_surface = t_cmb, t_surface
+        c = self.fv.cell_centers()
+        radius = torch.sqrt((c * c).sum(-1))
+        T = ((prm.r_max - radius) / (prm.r_max - prm.r_min)) ** 5
+        g = torch.Generator(device=T.device)
+        g.manual_seed(seed)
+        pert = noise * (2.0 * torch.rand(T.shape, generator=g, device=T.device, dtype=T.dtype) - 1.0)
+        real = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+        self.T = torch.zeros_like(T)
+        self.T[real] = torch.clamp(T[real] + pert[real], 0.0, 1.0)
+        self.fv.apply_dirichlet(self.T, cmb=t_cmb, surf=t_surface)
+        self.fv.ghost_update(self.T)
+        self.source = None
+        if internal_heating is not None:
+            self.source = torch.full_like(self.T, float(internal_heating))
+        self.T_nodal = torch.zeros(vel.N, dtype=torch.float64, device=T.device)
+        self.fv.to_fe(self.T, self.T_nodal)
+        self.table = []
+
+    def step(self, u):
+        """Energy step with the velocity u (FP64 nodal): returns the table row."""
+        torch = _torch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dt_stable = self.fv.dt_stable(u, self.kappa)
+        dt = self.dt_scaling * dt_stable
+        for _ in range(self.substeps):
+            self.fv.explicit_step(self.T, u, dt, self.kappa, self.source, True, cmb=self.t_cmb, surf=self.t_surface)
+            self.fv.apply_dirichlet(self.T, cmb=self.t_cmb, surf=self.t_surface)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        info = self.fv.to_fe(self.T, self.T_nodal)
+        torch.cuda.synchronize()
+        real = self.T[:, 1:-1, 1:-1, 1:-1]
+        row = {"dt": dt, "dt_stable": dt_stable, "fct_s": t1 - t0, "projection_iterations": info["iterations"],
+               "projection_s": time.perf_counter() - t1, "T_min": float(real.min()), "T_max": float(real.max())}
+        self.table.append(row)
+        return row
+
+
 def main():
     ap = argparse.ArgumentParser(description="Stokes solves of the mantle-circulation time step (warm-started)")
     ap.add_argument("--lat", type=int, default=4)
@@ -133,6 +197,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--free-slip", action="store_true", help="free-slip shells (mean-free pressure)")
     ap.add_argument("--prec", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--energy", action="store_true",
+                    help="couple the FCT energy step: T for each Stokes solve comes from the previous energy step")
     ap.add_argument("--xdmf", default=None, help="write an XDMF time series / checkpoint to this directory")
     a = ap.parse_args()
     torch = _torch()
@@ -151,15 +217,25 @@ def main():
         out = XDMFOutput(a.xdmf, app.pr.vel)
         out.add("velocity", lambda: app.u, vec_dim=3)
         out.add("temperature", lambda: Tk)
+    energy = MantleCirculationEnergy(app) if a.energy else None
     for k in range(a.steps):
-        wobble = 0.02 * torch.sin(3.0 * lat[:, 0] + 0.3 * k)[:, None].expand(-1, app.pr.vel.nr).reshape(-1)
-        Tk = T * (1.0 + wobble)
+        if energy is not None:
+            Tk = energy.T_nodal
+        else:
+            wobble = 0.02 * torch.sin(3.0 * lat[:, 0] + 0.3 * k)[:, None].expand(-1, app.pr.vel.nr).reshape(-1)
+            Tk = T * (1.0 + wobble)
         row = app.step(Tk)
+        msg = (f"timestep {row['timestep']}: {row['iterations']} FGMRES iterations, "
+               f"residual {row['initial_residual']:.3e} -> {row['final_residual']:.3e}, "
+               f"{row['step_s'] * 1e3:.1f} ms")
+        if energy is not None:
+            e = energy.step(app.u)
+            msg += (f"; energy dt {e['dt']:.3e}, FCT {e['fct_s'] * 1e3:.2f} ms, projection "
+                    f"{e['projection_iterations']} CG iterations {e['projection_s'] * 1e3:.1f} ms, "
+                    f"T in [{e['T_min']:.3f}, {e['T_max']:.3f}]")
         if out is not None:
             out.write()
-        print(f"timestep {row['timestep']}: {row['iterations']} FGMRES iterations, "
-              f"residual {row['initial_residual']:.3e} -> {row['final_residual']:.3e}, "
-              f"{row['step_s'] * 1e3:.1f} ms", flush=True)
+        print(msg, flush=True)
     print(app.table_text(), flush=True)
 
 

@@ -24,6 +24,14 @@
         pair, FP64 and FP32; free-slip on both shells includes the rotation
         penalty (its moment pass and the fused store)
 
+    python -m terraneo_amd.kernel_timing energy [--lat 7] [--reps 10]
+        the FCT energy step on the velocity mesh lateral level lat x
+        (2^lat + 1) radial nodes (FV cells 10 (2^lat)^3): stable step, the
+        predictor, limiter and correction stages (each with its ghost-layer
+        exchanges), the explicit step, and the FV -> FE projection (its
+        right-hand side, and the mass-matrix CG to 1e-12); velocity a rigid
+        rotation plus noise, diffusivity 1, Dirichlet T on the shells
+
     python -m terraneo_amd.kernel_timing slice TRACE.csv [--reps 10]
         cuts the rocprofv3 kernel trace (CSV) of a `solve` run at the
         bracketing sort launches and prints, per precision, the kernel time
@@ -159,6 +167,40 @@ def transfers(lat, reps):
         print(f"velocity level {lat}: {name}: {_time(fn, reps):.3f} ms per call", flush=True)
 
 
+def energy(lat, reps):
+    dev = torch.device("cuda:0")
+    m = api.Mesh(lat, lat, 0.55, 1.0)
+    fv = api.FV(m)
+    n, nr = m.n, m.nr
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    xyz = torch.from_numpy(m.lateral_coords()).to(dev)[:, :, :, None, :] * \
+        torch.from_numpy(m.radii()).to(dev)[None, None, None, :, None]
+    omega = torch.tensor([0.3, -0.2, 0.7], dtype=torch.float64, device=dev)
+    u = torch.cross(omega.expand_as(xyz), xyz, dim=-1).movedim(-1, 0).contiguous()
+    u += 0.05 * torch.rand(u.shape, generator=g, device=dev, dtype=torch.float64)
+    u = u.reshape(-1).contiguous()
+    T = torch.rand(fv.fv_shape, generator=g, device=dev, dtype=torch.float64)
+    fv.apply_dirichlet(T, cmb=1.0, surf=0.0)
+    kappa = 1.0
+    dt = 0.5 * fv.dt_stable(u, kappa)
+    cells = 10 * (n - 1) ** 2 * (nr - 1)
+    b = torch.empty(m.N, dtype=torch.float64, device=dev)
+    Tfe = torch.zeros(m.N, dtype=torch.float64, device=dev)
+    print(f"energy step, {cells} FV cells (lateral level {lat} x {nr} radial nodes), dt = {dt:.4e}", flush=True)
+    for name, fn in [("stable step (reduction)", lambda: fv.dt_stable(u, kappa)),
+                     ("predictor (+ ghosts of T)", lambda: fv.predictor(T, u, dt, kappa)),
+                     ("limiter (+ ghosts of T_L, R+, R-)", lambda: fv.limiter()),
+                     ("correction", lambda: fv.correction(T)),
+                     ("explicit step", lambda: fv.explicit_step(T, u, dt, kappa, None, True, cmb=1.0, surf=0.0)),
+                     ("FV -> FE right-hand side", lambda: fv.to_fe_rhs(T, b))]:
+        ms = _time(fn, reps)
+        print(f"  {name}: {ms:.3f} ms per call ({cells / ms / 1e6:.2f} Gcells/s)", flush=True)
+    info = fv.to_fe(T, Tfe)
+    print(f"  FV -> FE projection (CG on the mass matrix from 0 to 1e-12): {info['iterations']} iterations, "
+          f"{info['seconds'] * 1e3:.2f} ms", flush=True)
+
+
 def slice_trace(path, reps):
     """Per-iteration kernel costs of the two solves in a kernel trace of `solve`."""
     import collections
@@ -203,7 +245,7 @@ def slice_trace(path, reps):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["stored", "coupling", "eps", "solve", "transfers", "slice"])
+    ap.add_argument("what", choices=["stored", "coupling", "eps", "solve", "transfers", "energy", "slice"])
     ap.add_argument("trace", nargs="?", help="slice: rocprofv3 kernel trace CSV of a `solve` run")
     ap.add_argument("--lat", type=int, default=7)
     ap.add_argument("--reps", type=int, default=10)
@@ -213,7 +255,8 @@ def main():
             ap.error("slice needs the kernel trace CSV")
         slice_trace(a.trace, a.reps)
         return
-    {"stored": stored, "coupling": coupling, "eps": eps, "solve": solve, "transfers": transfers}[a.what](a.lat, a.reps)
+    {"stored": stored, "coupling": coupling, "eps": eps, "solve": solve, "transfers": transfers,
+     "energy": energy}[a.what](a.lat, a.reps)
 
 
 if __name__ == "__main__":

@@ -97,3 +97,37 @@ def test_app_warm_started_steps(cuda):
     ones = torch.ones_like(app.p)
     assert abs(pm.dot(app.p, ones, 1)) <= 1e-10 * np.sqrt(pm.dot(app.p, app.p, 1) * pm.owned_nodes)
     assert len(app.table) == 3 and "iterations" in app.table_text()
+
+
+@pytest.mark.gpu
+def test_time_loop_with_energy_step(cuda):
+    """Stokes + FCT energy time steps (mantlecirculation.cpp:866-1012): the
+    energy step's FCT update against the oracle for the step's own velocity,
+    temperatures inside [0, 1] up to rounding (local extremum diminishing with
+    Dirichlet shells 1 / 0), and the projected nodal temperature feeding the
+    next solve"""
+    import torch
+
+    import fv_oracle_lib as F
+    from terraneo_amd.app import AppParameters, MantleCirculationEnergy, MantleCirculationStokes
+
+    prm = AppParameters(lat=3, rad=3, levels=3)
+    app = MantleCirculationStokes(prm)
+    energy = MantleCirculationEnergy(app, diffusivity=0.01)
+    om = O.Mesh(3, 9, 0.5, 1.0)
+    co = F.cell_centers(om)
+    for k in range(3):
+        row = app.step(energy.T_nodal)
+        assert np.isfinite(row["final_residual"]) and row["final_residual"] < row["initial_residual"]
+        T0 = energy.T.cpu().numpy().copy()
+        u = app.u.cpu().numpy()
+        e = energy.step(app.u)
+        assert -1e-12 <= e["T_min"] and e["T_max"] <= 1.0 + 1e-12
+        assert abs(e["dt_stable"] - F.dt_stable(om, co, u, 0.01)) <= 1e-12 * e["dt_stable"]
+        st = F.FctStepper(om, co)
+        To = st.step(u, T0, e["dt"], 0.01, None, True, cmb=1.0, surf=0.0)
+        To = F.dirichlet(om, To, cmb=1.0, surf=0.0)
+        Tg = energy.T.cpu().numpy()
+        real = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+        assert np.abs(Tg[real] - To[real]).max() <= 1e-11
+        assert e["projection_iterations"] > 0
