@@ -16,6 +16,8 @@
 // written by one thread, so results repeat bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "tn_fv.hpp"
 
 namespace tn {
@@ -94,85 +96,101 @@ __device__ __forceinline__ void qpoint(int q, double& xi, double& eta, double& z
 // Geometry of real FV cell (d, xc, yc, rc): beta[f] = int_face u.n dS and
 // S[f] = int_face n dS (faces 0: x-1, 1: x+1, 2: y-1, 3: y+1, 4: r-1, 5: r+1),
 // each oriented towards the neighbour's centre, and the cell volume M.
-// Lateral faces: 2 x 2 Gauss points (lateral coordinate on [0, 1], weight
-// 1/2); radial faces: the 3-point triangle rule on both wedges.  The
-// wedge-local interpolation of u and of the node positions follows the
-// reference's shape functions.
+//
+// The reference integrates the faces with 2 x 2 Gauss points (lateral faces)
+// and the 3-point triangle rule on both wedges (radial faces), and the volume
+// with the Felippa 3x2 rule.  With the wedge map X = rho(zeta) L(xi, eta)
+// (rho linear from r1 to r2, L on the flat triangle V0 V1 V2, h = (r2-r1)/2)
+// these rules are exact, and the integrals have closed forms:
+//   lateral face xi = 0 of a wedge:  n = rho h (V2 x V0), nodes V0, V2
+//   lateral face eta = 0:            n = rho h (V1 x V0), nodes V0, V1
+//     S = h (r1 + r2) (Vb x Va),
+//     beta = h (Vb x Va) . [ab (u_a,bot + u_b,bot) + at (u_a,top + u_b,top)] / 2,
+//     ab = (2 r1 + r2) / 3, at = (r1 + 2 r2) / 3     (int rho (1 -+ zeta)/2 dzeta)
+//   radial face at radius rr (r1 bottom, r2 top), summed over both wedges:
+//     n = rr^2 (V1 - V0) x (V2 - V0),  S = n / 2,  beta = n . (u_0 + u_1 + u_2) / 6
+//   volume: sum over the wedges of |det(V0, V1, V2)| (r2^3 - r1^3) / 6
+// so the results equal the reference's quadrature up to rounding.
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
 __device__ void cell_geometry(const FvArgs& a, const double* __restrict__ vel, int64_t N, int d, int xc, int yc,
                               int rc, double beta[6], double& M, double S[6][3])
 {
     const int n = a.n, nr = a.nr;
-    double P[2][3][3];
-    wedge_corners(a.coords, n, d, xc - 1, yc - 1, P);
-    const double r1 = a.radii[rc - 1], r2 = a.radii[rc];
-    const double gl = 0.57735026918962584;  // 1/sqrt(3)
+    const int x = xc - 1, y = yc - 1, r = rc - 1;
+    // lateral corners: c00 = (x, y), c10 = (x+1, y), c01 = (x, y+1), c11 = (x+1, y+1)
+    double c[4][3];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        beta[f] = 0;
-        S[f][0] = S[f][1] = S[f][2] = 0;
+    for (int k = 0; k < 4; ++k) {
+        const double* p = a.coords + 3 * (((int64_t)d * n + x + (k & 1)) * n + y + (k >> 1));
+        c[k][0] = p[0];
+        c[k][1] = p[1];
+        c[k][2] = p[2];
     }
+    // nodal velocity at the 4 corners x {bottom, top}
+    double ub[4][3], ut[4][3];
 #pragma unroll
-    for (int w = 0; w < 2; ++w) {
-        double U[6][3];
+    for (int k = 0; k < 4; ++k) {
+        const int64_t node = (((int64_t)d * n + x + (k & 1)) * n + y + (k >> 1)) * nr + r;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            const int64_t node = (((int64_t)d * n + xc - 1 + FV_WOFF[w][i][0]) * n + yc - 1 + FV_WOFF[w][i][1]) * nr +
-                                 rc - 1 + FV_WOFF[w][i][2];
-            U[i][0] = vel[node];
-            U[i][1] = vel[N + node];
-            U[i][2] = vel[2 * N + node];
+        for (int j = 0; j < 3; ++j) {
+            ub[k][j] = vel[j * N + node];
+            ut[k][j] = vel[j * N + node + 1];
         }
-        // faces of this wedge: its two lateral quads, then the bottom and top triangles
-        const int fq0 = w == 0 ? 0 : 1, fq1 = w == 0 ? 2 : 3;
+    }
+    const double r1 = a.radii[r], r2 = a.radii[r + 1];
+    const double h = 0.5 * (r2 - r1);
+    const double ab = (2.0 * r1 + r2) / 3.0, at = (r1 + 2.0 * r2) / 3.0;
+    // wedge 0: V0 = c00, V1 = c10, V2 = c01; wedge 1: W0 = c11, W1 = c01, W2 = c10
+    // lateral faces (face, Va, Vb): 0 (c00, c01) xi = 0 of wedge 0: n ~ V2 x V0 = c01 x c00;
+    // 2 (c00, c10) eta = 0 of wedge 0: V1 x V0 = c10 x c00; 1 (c11, c10): W2 x W0 = c10 x c11;
+    // 3 (c11, c01): W1 x W0 = c01 x c11
+    const int FA[4] = {0, 3, 0, 3}, FB[4] = {2, 1, 1, 2};  // corner indices of faces 0, 1, 2, 3
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int f = k == 0 ? fq0 : (k == 1 ? fq1 : (k == 2 ? 4 : 5));
-            const bool tri = k >= 2;
-            const int np = tri ? 3 : 4;
-            for (int q = 0; q < np; ++q) {
-                double xi, eta, zeta, wq;
-                if (tri) {
-                    xi = q == 0 ? QT_A : QT_B;
-                    eta = q == 1 ? QT_A : QT_B;
-                    zeta = f == 4 ? -1.0 : 1.0;
-                    wq = QW;
-                } else {
-                    const double s = 0.5 * ((q < 2 ? -gl : gl) + 1.0), t = (q & 1) ? gl : -gl;
-                    wq = 0.5;
-                    zeta = t;
-                    if (k == 0) {  // x-faces: xi = 0, eta = s
-                        xi = 0;
-                        eta = s;
-                    } else {  // y-faces: eta = 0, xi = s
-                        xi = s;
-                        eta = 0;
-                    }
-                }
-                // dx/dxi, dx/deta, dx/dzeta and u at the point
-                const double lat[3] = {1.0 - xi - eta, xi, eta};
-                const double rb = 0.5 * (1 - zeta), rt = 0.5 * (1 + zeta);
-                double dxi[3], deta[3], dzeta[3], u[3];
+    for (int f = 0; f < 4; ++f) {
+        double nv[3];
+        cross3(c[FB[f]], c[FA[f]], nv);
+        const double sS = h * (r1 + r2);
+        double w = 0;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double xb0 = P[w][0][c] * r1, xb1 = P[w][1][c] * r1, xb2 = P[w][2][c] * r1;
-                    const double xt0 = P[w][0][c] * r2, xt1 = P[w][1][c] * r2, xt2 = P[w][2][c] * r2;
-                    dxi[c] = (xb1 - xb0) * rb + (xt1 - xt0) * rt;
-                    deta[c] = (xb2 - xb0) * rb + (xt2 - xt0) * rt;
-                    dzeta[c] = 0.5 * ((lat[0] * xt0 + lat[1] * xt1 + lat[2] * xt2) -
-                                      (lat[0] * xb0 + lat[1] * xb1 + lat[2] * xb2));
-                    u[c] = rb * (lat[0] * U[0][c] + lat[1] * U[1][c] + lat[2] * U[2][c]) +
-                           rt * (lat[0] * U[3][c] + lat[1] * U[4][c] + lat[2] * U[5][c]);
-                }
-                const double* e0 = tri ? dxi : (k == 0 ? deta : dxi);
-                const double* e1 = tri ? deta : dzeta;
-                const double nv[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2],
-                                      e0[0] * e1[1] - e0[1] * e1[0]};
-                beta[f] += wq * (u[0] * nv[0] + u[1] * nv[1] + u[2] * nv[2]);
-                S[f][0] += wq * nv[0];
-                S[f][1] += wq * nv[1];
-                S[f][2] += wq * nv[2];
-            }
-        }
+        for (int j = 0; j < 3; ++j)
+            w += nv[j] * (ab * (ub[FA[f]][j] + ub[FB[f]][j]) + at * (ut[FA[f]][j] + ut[FB[f]][j]));
+        beta[f] = 0.5 * h * w;
+        S[f][0] = sS * nv[0];
+        S[f][1] = sS * nv[1];
+        S[f][2] = sS * nv[2];
+    }
+    // radial faces: both wedges' (V1 - V0) x (V2 - V0)
+    double e1[3], e2[3], n0[3], n1[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        e1[j] = c[1][j] - c[0][j];
+        e2[j] = c[2][j] - c[0][j];
+    }
+    cross3(e1, e2, n0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        e1[j] = c[2][j] - c[3][j];
+        e2[j] = c[1][j] - c[3][j];
+    }
+    cross3(e1, e2, n1);
+    double bb = 0, bt = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        bb += n0[j] * (ub[0][j] + ub[1][j] + ub[2][j]) + n1[j] * (ub[3][j] + ub[2][j] + ub[1][j]);
+        bt += n0[j] * (ut[0][j] + ut[1][j] + ut[2][j]) + n1[j] * (ut[3][j] + ut[2][j] + ut[1][j]);
+    }
+    beta[4] = r1 * r1 * bb / 6.0;
+    beta[5] = r2 * r2 * bt / 6.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        S[4][j] = 0.5 * r1 * r1 * (n0[j] + n1[j]);
+        S[5][j] = 0.5 * r2 * r2 * (n0[j] + n1[j]);
     }
     // orientation: towards the neighbour's centre (ghost centres included)
     const double* cc = a.centers + 3 * fvi(n, nr, d, xc, yc, rc);
@@ -189,16 +207,13 @@ __device__ void cell_geometry(const FvArgs& a, const double* __restrict__ vel, i
             beta[f] = -beta[f];
         }
     }
-    double vol = 0;
-#pragma unroll
-    for (int w = 0; w < 2; ++w)
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            double xi, eta, zeta;
-            qpoint(q, xi, eta, zeta);
-            vol += abs_det(P[w], r1, r2, xi, eta, zeta) * QW;
-        }
-    M = vol;
+    // volume: |det(V0, V1, V2)| per wedge (= V0 . (V1 x V2))
+    double v12[3], w12[3];
+    cross3(c[1], c[2], v12);
+    cross3(c[2], c[1], w12);
+    const double det0 = fabs(c[0][0] * v12[0] + c[0][1] * v12[1] + c[0][2] * v12[2]);
+    const double det1 = fabs(c[3][0] * w12[0] + c[3][1] * w12[1] + c[3][2] * w12[2]);
+    M = (det0 + det1) * (r2 - r1) * (r1 * r1 + r1 * r2 + r2 * r2) / 6.0;  // (r2^3 - r1^3) / 6 without cancellation
 }
 
 __device__ __forceinline__ int64_t neighbour(int n, int nr, int d, int xc, int yc, int rc, int f)
@@ -263,9 +278,9 @@ __global__ __launch_bounds__(FV_BLOCK) void k_fv_dt(FvArgs a, const double* __re
     const int n = a.n, nr = a.nr;
     const int64_t cells = 10LL * (n - 1) * (n - 1) * (nr - 1);
     const int64_t N = 10LL * n * n * nr;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double dtc = 1e300;
-    if (t < cells) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cells;
+         t += (int64_t)gridDim.x * blockDim.x) {
         int d, xc, yc, rc;
         cell_real(t, n, nr, d, xc, yc, rc);
         double beta[6], M, S[6][3];
@@ -281,7 +296,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_fv_dt(FvArgs a, const double* __re
                 if (den > 0) lambda += kappa * (S[f][0] * S[f][0] + S[f][1] * S[f][1] + S[f][2] * S[f][2]) / den;
             }
         }
-        dtc = lambda > 0 ? M / lambda : 1e30;
+        dtc = fmin(dtc, lambda > 0 ? M / lambda : 1e30);
     }
     // block minimum (order-independent)
     __shared__ double red[FV_BLOCK / 64];
@@ -425,7 +440,10 @@ __global__ __launch_bounds__(FV_BLOCK) void k_fct_correction(int n, int nr, cons
 
 // right-hand side of the FV -> FE L2 projection (conversion.hpp:151-200):
 // b_i = sum over the cells K around node i of T_K int_K phi_i, gathered per
-// node copy (no atomics); the interface sum follows
+// node copy (no atomics); the interface sum follows.  The reference evaluates
+// int_K phi_i |det J| with the Felippa 3x2 rule, which is exact: per wedge
+//   |det(V0, V1, V2)| / 6 x (I_bot or I_top),  I = int rho^2 x radial hat drho,
+//   I_bot = dr (r1^2/2 + r1 dr/3 + dr^2/12),  I_top = dr (r1^2/2 + 2 r1 dr/3 + dr^2/4)
 __global__ __launch_bounds__(FV_BLOCK) void k_fv_to_fe_rhs(FvArgs a, const double* __restrict__ Tfv,
                                                            double* __restrict__ b)
 {
@@ -444,28 +462,22 @@ __global__ __launch_bounds__(FV_BLOCK) void k_fv_to_fe_rhs(FvArgs a, const doubl
             if (cy < 0 || cy > n - 2) continue;
             double P[2][3][3];
             wedge_corners(a.coords, n, d, cx, cy, P);
+            double det[2];
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                double v12[3];
+                cross3(P[w][1], P[w][2], v12);
+                det[w] = fabs(P[w][0][0] * v12[0] + P[w][0][1] * v12[1] + P[w][0][2] * v12[2]) / 6.0;
+            }
+            // the node's weight in this quad: the wedges that contain its lateral corner
+            const int lx = x - cx, ly = y - cy;
+            const double wlat = (lx == 0 && ly == 0) ? det[0] : ((lx == 1 && ly == 1) ? det[1] : det[0] + det[1]);
             for (int cr = r - 1; cr <= r; ++cr) {
                 if (cr < 0 || cr > nr - 2) continue;
-                const double r1 = a.radii[cr], r2 = a.radii[cr + 1];
-                const double tv = Tfv[fvi(n, nr, d, cx + 1, cy + 1, cr + 1)];
-                for (int w = 0; w < 2; ++w) {
-                    int li = -1;
-#pragma unroll
-                    for (int i = 0; i < 6; ++i)
-                        if (FV_WOFF[w][i][0] == x - cx && FV_WOFF[w][i][1] == y - cy && FV_WOFF[w][i][2] == r - cr)
-                            li = i;
-                    if (li < 0) continue;
-                    double s = 0;
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) {
-                        double xi, eta, zeta;
-                        qpoint(q, xi, eta, zeta);
-                        const double lat[3] = {1.0 - xi - eta, xi, eta};
-                        const double sh = lat[li % 3] * (li < 3 ? 0.5 * (1 - zeta) : 0.5 * (1 + zeta));
-                        s += sh * abs_det(P[w], r1, r2, xi, eta, zeta) * QW;
-                    }
-                    acc += s * tv;
-                }
+                const double r1 = a.radii[cr], dr = a.radii[cr + 1] - r1;
+                const double irad = (r == cr + 1) ? dr * (0.5 * r1 * r1 + r1 * dr * (2.0 / 3.0) + 0.25 * dr * dr)
+                                                  : dr * (0.5 * r1 * r1 + r1 * dr / 3.0 + dr * dr / 12.0);
+                acc += wlat * irad * Tfv[fvi(n, nr, d, cx + 1, cy + 1, cr + 1)];
             }
         }
     }
@@ -478,7 +490,9 @@ int blocks_for(int64_t work) { return (int)((work + FV_BLOCK - 1) / FV_BLOCK); }
 
 int64_t fv_real_cells(int n, int nr) { return 10LL * (n - 1) * (n - 1) * (nr - 1); }
 int64_t fv_size(int n, int nr) { return 10LL * (n + 1) * (n + 1) * (nr + 1); }
-int fv_dt_blocks(int n, int nr) { return blocks_for(fv_real_cells(n, nr)); }
+// grid-stride launch of the stable-step kernel: at most FV_DT_BLOCKS partial minima
+constexpr int FV_DT_BLOCKS = 2048;
+int fv_dt_blocks(int n, int nr) { return (int)std::min<int64_t>(blocks_for(fv_real_cells(n, nr)), FV_DT_BLOCKS); }
 
 void launch_fv_centers(const FvArgs& a, double* c, hipStream_t s)
 {

@@ -507,9 +507,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3))) voi
     march::run<T, W>(op, a.nf, a.nrf, a.seg, a.guard);
 }
 
-// VectorMass (ncomp components) / lumped k-weighted mass (LUMPED_K).
-template <typename T, bool LUMPED_K>
-__global__ __launch_bounds__(128) void k_mass(MassArgs<T> a)
+// lumped k-weighted mass (kmass.hpp): row sums of the k-weighted wedge mass
+// matrices, accumulated per wedge (setup only)
+template <typename T>
+__global__ __launch_bounds__(128) void k_kmass_lumped(MassArgs<T> a)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int ncr = a.nr - 1, ncl = a.n - 1;
@@ -544,30 +545,89 @@ __global__ __launch_bounds__(128) void k_mass(MassArgs<T> a)
             const double rr = G.r1 + 0.5 * (G.r2 - G.r1) * (1 + qp[2]);
             double N6[6];
             for (int j = 0; j < 6; ++j) N6[j] = shape_lat(j, qp[0], qp[1]) * shape_rad(j, qp[2]);
-            double kq = 1.0;
-            if (LUMPED_K) {
-                kq = 0.0;
-                for (int j = 0; j < 6; ++j) kq += N6[j] * (double)a.k[idx[j]];
-            }
+            double kq = 0.0;
+            for (int j = 0; j < 6; ++j) kq += N6[j] * (double)a.k[idx[j]];
             const double f = Q6W * kq * rr * rr * gr * det;
             for (int i = 0; i < 6; ++i)
                 for (int j = 0; j < 6; ++j) M[i][j] += f * N6[i] * N6[j];
         }
-        if (LUMPED_K) {
-            for (int i = 0; i < 6; ++i) {
-                double s = 0.0;
-                for (int j = 0; j < 6; ++j) s += M[i][j];
-                unsafeAtomicAdd(a.y + idx[i], (T)s);
-            }
-        } else {
-            for (int c = 0; c < a.ncomp; ++c)
-                for (int i = 0; i < 6; ++i) {
-                    double s = 0.0;
-                    for (int j = 0; j < 6; ++j) s += M[i][j] * (double)a.x[c * N + idx[j]];
-                    unsafeAtomicAdd(a.y + c * N + idx[i], (T)s);
-                }
+        for (int i = 0; i < 6; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < 6; ++j) s += M[i][j];
+            unsafeAtomicAdd(a.y + idx[i], (T)s);
         }
     }
+}
+
+// VectorMass as a gather: one thread per node copy writes y_i = sum over the
+// wedges of its diamond that contain it of (M_w x)_i; no atomics, so the
+// result repeats bit for bit.  The reference's Felippa 3x2 rule factors into
+// the 3-point triangle rule (exact for the lateral products: the P1 triangle
+// mass (1 + delta_ij) / 24 times |det(V0, V1, V2)|) and the 2-point Gauss rule
+// in zeta for rho^2 hat_a hat_b (not exact: degree 4), which is kept:
+//   M_w[i][j] = |det_w| (1 + delta_ij) / 24 x h sum_{g = -+1/sqrt 3} rho(g)^2 hat_a(g) hat_b(g)
+template <typename T>
+__global__ __launch_bounds__(256) void k_mass_gather(MassArgs<T> a)
+{
+    const int n = a.n, nr = a.nr;
+    const int64_t N = 10LL * n * n * nr;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int r = (int)(t % nr);
+    const int y = (int)((t / nr) % n);
+    const int x = (int)((t / ((int64_t)nr * n)) % n);
+    const int d = (int)(t / ((int64_t)nr * n * n));
+    constexpr double g = 0.57735026918962584;  // 1/sqrt(3)
+    double acc[3] = {0, 0, 0};
+    for (int cr = r - 1; cr <= r; ++cr) {
+        if (cr < 0 || cr > nr - 2) continue;
+        // radial 2x2 factor of this layer: a = the node's level, b = the other's
+        const double r1 = a.radii[cr], r2 = a.radii[cr + 1], h = 0.5 * (r2 - r1);
+        const double rm = r1 + h * (1.0 - g), rp = r1 + h * (1.0 + g);  // rho at zeta = -g, +g
+        const double bm = 0.5 * (1.0 + g), bp = 0.5 * (1.0 - g);        // bottom hat at -g, +g
+        const double Rbb = h * (rm * rm * bm * bm + rp * rp * bp * bp);
+        const double Rtt = h * (rm * rm * bp * bp + rp * rp * bm * bm);
+        const double Rbt = h * (rm * rm * bm * bp + rp * rp * bp * bm);
+        const bool top = r == cr + 1;  // the node is the layer's top level
+        const double Rs = top ? Rtt : Rbb;
+        const int ro = top ? cr + 1 : cr, rq = top ? cr : cr + 1;  // own level, other level
+        for (int cx = x - 1; cx <= x; ++cx) {
+            if (cx < 0 || cx > n - 2) continue;
+            for (int cy = y - 1; cy <= y; ++cy) {
+                if (cy < 0 || cy > n - 2) continue;
+                for (int w = 0; w < 2; ++w) {
+                    int li = -1;
+                    for (int k = 0; k < 3; ++k)
+                        if (WOFF[w][k][0] == x - cx && WOFF[w][k][1] == y - cy) li = k;
+                    if (li < 0) continue;
+                    int64_t lat[3];
+                    double P[3][3];
+                    for (int k = 0; k < 3; ++k) {
+                        const int xx = cx + WOFF[w][k][0], yy = cy + WOFF[w][k][1];
+                        lat[k] = ((int64_t)d * n + xx) * n + yy;
+                        const double* c = a.coords + 3 * lat[k];
+                        P[k][0] = c[0];
+                        P[k][1] = c[1];
+                        P[k][2] = c[2];
+                    }
+                    const double det = fabs(P[0][0] * (P[1][1] * P[2][2] - P[1][2] * P[2][1]) -
+                                            P[0][1] * (P[1][0] * P[2][2] - P[1][2] * P[2][0]) +
+                                            P[0][2] * (P[1][0] * P[2][1] - P[1][1] * P[2][0])) /
+                                       24.0;
+                    for (int c = 0; c < a.ncomp; ++c) {
+                        double s = 0;
+                        for (int k = 0; k < 3; ++k) {
+                            const double lm = det * (k == li ? 2.0 : 1.0);
+                            s += lm * (Rs * (double)a.x[c * N + lat[k] * nr + ro] +
+                                       Rbt * (double)a.x[c * N + lat[k] * nr + rq]);
+                        }
+                        acc[c] += s;
+                    }
+                }
+            }
+        }
+    }
+    for (int c = 0; c < a.ncomp; ++c) a.y[c * N + t] = (T)acc[c];
 }
 
 }  // namespace
@@ -601,12 +661,14 @@ void launch_grad(const GradArgs<T>& a, hipStream_t s)
 template <typename T>
 void launch_mass(const MassArgs<T>& a, bool lumped_k, hipStream_t s)
 {
-    const int64_t ncell = 10LL * (a.n - 1) * (a.n - 1) * (a.nr - 1);
-    const dim3 gr((unsigned)((ncell + 127) / 128)), bl(128);
-    if (lumped_k)
-        hipLaunchKernelGGL((k_mass<T, true>), gr, bl, 0, s, a);
-    else
-        hipLaunchKernelGGL((k_mass<T, false>), gr, bl, 0, s, a);
+    if (lumped_k) {
+        const int64_t ncell = 10LL * (a.n - 1) * (a.n - 1) * (a.nr - 1);
+        hipLaunchKernelGGL((k_kmass_lumped<T>), dim3((unsigned)((ncell + 127) / 128)), dim3(128), 0, s, a);
+    } else {
+        TN_REQUIRE(a.ncomp >= 1 && a.ncomp <= 3, "mass: ncomp must be 1..3");
+        const int64_t N = 10LL * a.n * a.n * a.nr;
+        hipLaunchKernelGGL((k_mass_gather<T>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
+    }
     TN_HIP(hipGetLastError());
 }
 

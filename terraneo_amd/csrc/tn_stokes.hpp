@@ -51,7 +51,7 @@ struct MassArgs {
     int n, nr, ncomp;
     const T* x;  // input (vector mass)
     const T* k;  // coefficient (lumped k-mass)
-    T* y;        // accumulated output
+    T* y;        // output: written (vector mass), accumulated (lumped k-mass)
 };
 
 template <typename T>
