@@ -5,7 +5,9 @@ forward:  m [10][n-1][n-1][nr-1] -> eta_cell = exp(m) -> solve K(m) (u, p) = b
 backward: given dJ/du (torch's Euclidean gradient over the stored node
           copies), form its owned-inner-product representative (sum over the
           copies of every node), solve the adjoint system K (lu, lp) = (g, 0)
-          with the same solver (K is symmetric), then
+          with the same solver (K is symmetric; on free-slip shells with the
+          penalty once per node the rigid-rotation part of g is removed first,
+          which leaves the gradient unchanged), then
           dJ/dm[cell] = -eta_cell (T lu)^T A_cell(1) u      (library kernel)
 
 The reference has no adjoint; the discrete gradient is exact for the discrete
@@ -50,6 +52,13 @@ def make_function():
                     from . import api
 
                     api.freeslip_project(problem.vel, g, which)
+            if problem.bc == (TN_FREESLIP, TN_FREESLIP) and problem.penalty == "once":
+                # rigid rotations: an invariant subspace of the symmetric K without
+                # strain, so their part of g cannot change the gradient; removing it
+                # spares the adjoint solve its penalty-scaled, slowly converging part
+                from .forward_adjoint import remove_rotations
+
+                g = remove_rotations(problem, g)
             gp = torch.zeros(problem.pre.N, dtype=torch.float64, device=u.device)
             lu, _, info = problem.solve(g, gp, **ctx.solve_kw)
             ctx.adjoint_info = info

@@ -474,7 +474,7 @@ int tn_fv_to_fe(tn_fv* fv, const double* Tfv, double* Tfe, const tn_krylov_optio
         {
             double* yy = (double*)y.b[0].p;
             launch_fill<double>(yy, L->N, 0.0, c.s);
-            MassArgs<double> a{L->d_coords.p, L->d_radii.p, L->n, L->nr, 1, (const double*)x.b[0].p, nullptr, yy};
+            MassArgs<double> a{L->d_coords.p, L->d_radii.p, L->n, L->nr, 1, (const double*)x.b[0].p, nullptr, yy, L->d_geo.p};
             launch_mass<double>(a, false, c.s);
             launch_interface_sum<double>(*L, 1, yy, c.s);
         }

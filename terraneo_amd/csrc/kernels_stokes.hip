@@ -562,14 +562,18 @@ __global__ __launch_bounds__(128) void k_kmass_lumped(MassArgs<T> a)
 // VectorMass as a gather: one thread per node copy writes y_i = sum over the
 // wedges of its diamond that contain it of (M_w x)_i; no atomics, so the
 // result repeats bit for bit.  The reference's Felippa 3x2 rule factors into
-// the 3-point triangle rule (exact for the lateral products: the P1 triangle
-// mass (1 + delta_ij) / 24 times |det(V0, V1, V2)|) and the 2-point Gauss rule
-// in zeta for rho^2 hat_a hat_b (not exact: degree 4), which is kept:
-//   M_w[i][j] = |det_w| (1 + delta_ij) / 24 x h sum_{g = -+1/sqrt 3} rho(g)^2 hat_a(g) hat_b(g)
+// the 3-point triangle rule, exact for the lateral products (the P1 triangle
+// mass (1 + delta_ij) / 24 times |det(V0, V1, V2)|), and the 2-point Gauss
+// rule in zeta for rho^2 hat_a hat_b, which is not exact (degree 4) and is kept:
+//   M_w[i][j] = |det_w| (1 + delta_ij) / 24 x h sum_{g = -+1/sqrt 3} rho(g)^2 hat_a(g) hat_b(g).
+// Summed over the six wedges around a lateral node this is a 7-point lateral
+// stencil (the node and its neighbours (x+-1, y), (x, y+-1), (x-1, y+1),
+// (x+1, y-1)) times a 3-point radial stencil; |det_w| comes from the mesh's
+// wedge table (geo[..][9]).
 template <typename T>
-__global__ __launch_bounds__(256) void k_mass_gather(MassArgs<T> a)
+__global__ __launch_bounds__(256) void k_mass_stencil(MassArgs<T> a)
 {
-    const int n = a.n, nr = a.nr;
+    const int n = a.n, nr = a.nr, nc = n - 1;
     const int64_t N = 10LL * n * n * nr;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
@@ -577,54 +581,45 @@ __global__ __launch_bounds__(256) void k_mass_gather(MassArgs<T> a)
     const int y = (int)((t / nr) % n);
     const int x = (int)((t / ((int64_t)nr * n)) % n);
     const int d = (int)(t / ((int64_t)nr * n * n));
+    auto det = [&](int qx, int qy, int w) -> double {
+        if (qx < 0 || qy < 0 || qx >= nc || qy >= nc) return 0.0;
+        return a.geo[((((int64_t)d * nc + qx) * nc + qy) * 2 + w) * GEO_STRIDE + 9];
+    };
+    // the six wedges around (x, y): quad (x, y) w0; (x-1, y) w0, w1; (x, y-1) w0, w1; (x-1, y-1) w1
+    const double a0 = det(x, y, 0), b0 = det(x - 1, y, 0), b1 = det(x - 1, y, 1);
+    const double c0 = det(x, y - 1, 0), c1 = det(x, y - 1, 1), d1 = det(x - 1, y - 1, 1);
+    constexpr double s = 1.0 / 24.0;
+    constexpr int NB = 7;
+    const int ox[NB] = {0, 1, 0, -1, 0, -1, 1}, oy[NB] = {0, 0, 1, 0, -1, 1, -1};
+    const double wl[NB] = {2.0 * s * (a0 + b0 + b1 + c0 + c1 + d1), s * (a0 + c1), s * (a0 + b1),
+                           s * (b0 + d1),                          s * (c0 + d1), s * (b0 + b1), s * (c0 + c1)};
+    // radial stencil: the node's own level and the levels below / above
     constexpr double g = 0.57735026918962584;  // 1/sqrt(3)
+    constexpr double hm = 0.5 * (1.0 + g), hp = 0.5 * (1.0 - g);  // bottom hat at zeta = -g, +g
+    double cm = 0.0, c0r = 0.0, cp = 0.0;
+    if (r > 0) {  // layer (r-1, r): the node is its top level
+        const double r1 = a.radii[r - 1], h = 0.5 * (a.radii[r] - r1);
+        const double rm = r1 + h * (1.0 - g), rp = r1 + h * (1.0 + g);
+        c0r += h * (rm * rm * hp * hp + rp * rp * hm * hm);
+        cm = h * (rm * rm + rp * rp) * hm * hp;
+    }
+    if (r < nr - 1) {  // layer (r, r+1): the node is its bottom level
+        const double r1 = a.radii[r], h = 0.5 * (a.radii[r + 1] - r1);
+        const double rm = r1 + h * (1.0 - g), rp = r1 + h * (1.0 + g);
+        c0r += h * (rm * rm * hm * hm + rp * rp * hp * hp);
+        cp = h * (rm * rm + rp * rp) * hm * hp;
+    }
     double acc[3] = {0, 0, 0};
-    for (int cr = r - 1; cr <= r; ++cr) {
-        if (cr < 0 || cr > nr - 2) continue;
-        // radial 2x2 factor of this layer: a = the node's level, b = the other's
-        const double r1 = a.radii[cr], r2 = a.radii[cr + 1], h = 0.5 * (r2 - r1);
-        const double rm = r1 + h * (1.0 - g), rp = r1 + h * (1.0 + g);  // rho at zeta = -g, +g
-        const double bm = 0.5 * (1.0 + g), bp = 0.5 * (1.0 - g);        // bottom hat at -g, +g
-        const double Rbb = h * (rm * rm * bm * bm + rp * rp * bp * bp);
-        const double Rtt = h * (rm * rm * bp * bp + rp * rp * bm * bm);
-        const double Rbt = h * (rm * rm * bm * bp + rp * rp * bp * bm);
-        const bool top = r == cr + 1;  // the node is the layer's top level
-        const double Rs = top ? Rtt : Rbb;
-        const int ro = top ? cr + 1 : cr, rq = top ? cr : cr + 1;  // own level, other level
-        for (int cx = x - 1; cx <= x; ++cx) {
-            if (cx < 0 || cx > n - 2) continue;
-            for (int cy = y - 1; cy <= y; ++cy) {
-                if (cy < 0 || cy > n - 2) continue;
-                for (int w = 0; w < 2; ++w) {
-                    int li = -1;
-                    for (int k = 0; k < 3; ++k)
-                        if (WOFF[w][k][0] == x - cx && WOFF[w][k][1] == y - cy) li = k;
-                    if (li < 0) continue;
-                    int64_t lat[3];
-                    double P[3][3];
-                    for (int k = 0; k < 3; ++k) {
-                        const int xx = cx + WOFF[w][k][0], yy = cy + WOFF[w][k][1];
-                        lat[k] = ((int64_t)d * n + xx) * n + yy;
-                        const double* c = a.coords + 3 * lat[k];
-                        P[k][0] = c[0];
-                        P[k][1] = c[1];
-                        P[k][2] = c[2];
-                    }
-                    const double det = fabs(P[0][0] * (P[1][1] * P[2][2] - P[1][2] * P[2][1]) -
-                                            P[0][1] * (P[1][0] * P[2][2] - P[1][2] * P[2][0]) +
-                                            P[0][2] * (P[1][0] * P[2][1] - P[1][1] * P[2][0])) /
-                                       24.0;
-                    for (int c = 0; c < a.ncomp; ++c) {
-                        double s = 0;
-                        for (int k = 0; k < 3; ++k) {
-                            const double lm = det * (k == li ? 2.0 : 1.0);
-                            s += lm * (Rs * (double)a.x[c * N + lat[k] * nr + ro] +
-                                       Rbt * (double)a.x[c * N + lat[k] * nr + rq]);
-                        }
-                        acc[c] += s;
-                    }
-                }
-            }
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        if (wl[k] == 0.0) continue;  // a neighbour outside the diamond (no wedge holds it)
+        const int64_t col = (((int64_t)d * n + x + ox[k]) * n + y + oy[k]) * nr + r;
+        for (int c = 0; c < a.ncomp; ++c) {
+            const T* xc = a.x + c * N + col;
+            double v = c0r * (double)xc[0];
+            if (r > 0) v += cm * (double)xc[-1];
+            if (r < nr - 1) v += cp * (double)xc[1];
+            acc[c] += wl[k] * v;
         }
     }
     for (int c = 0; c < a.ncomp; ++c) a.y[c * N + t] = (T)acc[c];
@@ -667,7 +662,8 @@ void launch_mass(const MassArgs<T>& a, bool lumped_k, hipStream_t s)
     } else {
         TN_REQUIRE(a.ncomp >= 1 && a.ncomp <= 3, "mass: ncomp must be 1..3");
         const int64_t N = 10LL * a.n * a.n * a.nr;
-        hipLaunchKernelGGL((k_mass_gather<T>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
+        TN_REQUIRE(a.geo, "mass: wedge table missing");
+        hipLaunchKernelGGL((k_mass_stencil<T>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
     }
     TN_HIP(hipGetLastError());
 }

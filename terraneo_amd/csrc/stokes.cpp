@@ -215,7 +215,7 @@ int tn_vector_mass_apply(const tn_mesh* m, int dtype, const void* x, void* y, tn
     auto go = [&](auto* yy) {
         using T = std::remove_pointer_t<decltype(yy)>;
         tn::launch_fill<T>(yy, 3 * L.N, T(0), (hipStream_t)s);
-        tn::MassArgs<T> a{L.d_coords.p, L.d_radii.p, L.n, L.nr, 3, (const T*)x, nullptr, yy};
+        tn::MassArgs<T> a{L.d_coords.p, L.d_radii.p, L.n, L.nr, 3, (const T*)x, nullptr, yy, L.d_geo.p};
         tn::launch_mass<T>(a, false, (hipStream_t)s);
         tn::launch_interface_sum<T>(L, 3, yy, (hipStream_t)s);
     };
@@ -346,7 +346,7 @@ int tn_stokes_rhs_from_temperature(tn_stokes* K, const void* temperature, double
         TN_HIP(hipGetLastError());
         // f_u = M g, additive communication (vector_mass.hpp:59-153)
         tn::launch_fill<T>(fu, 3 * L.N, T(0), st);
-        tn::MassArgs<T> a{L.d_coords.p, L.d_radii.p, L.n, L.nr, 3, g, nullptr, fu};
+        tn::MassArgs<T> a{L.d_coords.p, L.d_radii.p, L.n, L.nr, 3, g, nullptr, fu, L.d_geo.p};
         tn::launch_mass<T>(a, false, st);
         tn::launch_interface_sum<T>(L, 3, fu, st);
         // strong enforcement of the velocity BCs on the right-hand side

@@ -52,6 +52,7 @@ struct MassArgs {
     const T* x;  // input (vector mass)
     const T* k;  // coefficient (lumped k-mass)
     T* y;        // output: written (vector mass), accumulated (lumped k-mass)
+    const double* geo = nullptr;  // the mesh's wedge table [10][n-1][n-1][2][GEO_STRIDE] (vector mass)
 };
 
 template <typename T>

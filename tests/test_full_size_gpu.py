@@ -200,3 +200,34 @@ def test_chunked_solve_equals_single_call(cuda):
     dp = float((p1 - p2).abs().max() / p1.abs().max())
     print(f"chunked vs single call: {c['iterations']} iterations, velocity {du:.2e} pressure {dp:.2e}", flush=True)
     assert du <= 1e-9 and dp <= 1e-9
+
+
+@pytest.mark.gpu
+def test_adjoint_rotation_part_does_not_change_the_gradient(cuda):
+    """fs/fs with the penalty once per node: the rigid rotations are an invariant
+    subspace of K and carry no strain, so removing the adjoint right-hand side's
+    rotation part leaves the viscosity gradient unchanged (lateral 4, FP64
+    V-cycle, the lateral-8 workload's viscosity)"""
+    import torch
+
+    from terraneo_amd.forward_adjoint import build_problem, remove_rotations, rotation_content, surface_objective
+
+    pr, m, bu, bp, coords, radii = build_problem(4, 4, 3, prec_dtype="f64")
+    kw = dict(rtol=1e-12, max_iterations=4000, restart=30, pressure_mean_free=True)
+    u, p, info = pr.solve(bu, bp, **kw)
+    J, g = surface_objective(pr, u)
+    rot = rotation_content(pr, g)
+    gp = torch.zeros(pr.pre.N, dtype=torch.float64, device=cuda)
+    grads, its = [], []
+    for rhs in (g, remove_rotations(pr, g)):
+        lam, _, ai = pr.solve(rhs, gp, **kw)
+        gr = torch.empty_like(m)
+        pr.A.grad_log_cell_eta(u, lam, gr)
+        grads.append(gr)
+        its.append((ai["iterations"], ai["final_residual"] / ai["initial_residual"]))
+    diff = float((grads[0] - grads[1]).abs().max() / grads[0].abs().max())
+    print(f"rotation content {rot:.3e}; adjoint (iterations, rel. res) with / without it {its}; "
+          f"gradient difference {diff:.2e}", flush=True)
+    assert rot > 1e-3
+    assert its[1][1] <= 1e-12 * (1 + 1e-9)
+    assert diff <= 1e-6
