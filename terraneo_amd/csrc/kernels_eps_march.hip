@@ -552,6 +552,9 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
     const __amdgpu_buffer_rsrc_t rs_eta = buf_rsrc(nodal ? (const void*)a.eta : (const void*)a.src, nodal ? Nb : 0u);
     const int bmode = a.bc_cmb, tmode = a.bc_surf;
     const int nchunks = (ncr + 63) >> 6;
+    // fs/fs rotation penalty y += gamma x p, gamma = S beta (FP64, formed by the
+    // moment pass's finalize, which runs before this kernel on the same stream)
+    const double pg0 = PEN ? pen[13] : 0.0, pg1 = PEN ? pen[14] : 0.0, pg2 = PEN ? pen[15] : 0.0;
     // byte offsets of radial columns: node (xc, yc) of this diamond at ((d n + xc) n + yc) nr
     const uint32_t colstep = (uint32_t)nr * SZ;             // next y
     const uint32_t rowstep = (uint32_t)n * nr * SZ;         // next x
@@ -570,6 +573,9 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
         const T r0 = a.radii[cc], r1 = a.radii[cc + 1];
         const T half_dr = T(0.5) * (r1 - r0), r_mid = T(0.5) * (r0 + r1);
         const T gam = r_mid / half_dr, hd6 = half_dr * T(1.0 / 6.0);
+        // radius of the lane's node level and of the chunk's top level (fs/fs penalty, FP64)
+        const double rad_lane = PEN ? a.radii_d[cc] : 0.0;
+        const double rad_top = PEN ? a.radii_d[64 * h + toplane + 1] : 0.0;
         // boundary lanes of this chunk (trial side on load, test side at the store)
         const bool bot_b = BND && h == 0 && bmode != TN_NEUMANN && lane == 0;
         const bool top_b = BND && last && tmode != TN_NEUMANN && c == ncr - 1;
@@ -633,6 +639,16 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
             const int step = step0 + (qy - qs);
             T(*E)[65] = ebuf[step % RING][w];
             T yb[4][3], yt[4][3];
+            // fs/fs penalty: unit coordinates of the node (x, qy) stored by this
+            // step, loaded (wave-uniform address) before the cell arithmetic hides
+            // their latency
+            double pc0 = 0.0, pc1 = 0.0, pc2 = 0.0;
+            if (PEN && row_own && qy >= y0) {
+                const int o = __builtin_amdgcn_readfirstlane(3 * ((d * n + x) * n + qy));
+                pc0 = coords_d[o];
+                pc1 = coords_d[o + 1];
+                pc2 = coords_d[o + 2];
+            }
             if (quad) {
                 const uint32_t cb = colx + (uint32_t)(qy + 1) * colstep;
                 if (!PF) {
@@ -700,15 +716,11 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
                             v = test_value<T, false>(v, xv, mode, coords, lat, blk);
                         }
                         if (PEN) {
-                            // fs/fs rotation penalty y += gamma x p, gamma = S beta (FP64, formed
-                            // by the moment pass's finalize)
-                            const double g0 = pen[13], g1 = pen[14], g2 = pen[15];
-                            const double r = a.radii_d[level];
-                            const double px = coords_d[3 * lat] * r, py = coords_d[3 * lat + 1] * r,
-                                         pz = coords_d[3 * lat + 2] * r;
-                            v.x += (T)(g1 * pz - g2 * py);
-                            v.y += (T)(g2 * px - g0 * pz);
-                            v.z += (T)(g0 * py - g1 * px);
+                            const double r = level == c ? rad_lane : rad_top;
+                            const double px = pc0 * r, py = pc1 * r, pz = pc2 * r;
+                            v.x += (T)(pg1 * pz - pg2 * py);
+                            v.y += (T)(pg2 * px - pg0 * pz);
+                            v.z += (T)(pg0 * py - pg1 * px);
                         }
                     }
                     const uint32_t lo = (uint32_t)level * SZ;
