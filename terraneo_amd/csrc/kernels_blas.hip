@@ -157,14 +157,17 @@ __global__ __launch_bounds__(256) void k_cheby_update(T* __restrict__ x, T* __re
 }
 
 // w += alpha x, then the per-block sums of sum_owned w . y (y == w: the new w's squared norm):
-// one modified Gram-Schmidt step fused with the next step's dot.
+// one modified Gram-Schmidt step fused with the next step's dot.  With alpha_dev
+// the coefficient is alpha * alpha_dev[0], read on the device (a preceding
+// reduction's result), so the host does not wait for it.
 template <typename T, bool SELF>
-__global__ __launch_bounds__(256) void k_axpy_dot(T* __restrict__ w, double alpha, const T* __restrict__ x,
-                                                  const T* __restrict__ y, const uint8_t* __restrict__ owned_lat,
-                                                  int64_t N, int nr, int ncomp, double* __restrict__ part)
+__global__ __launch_bounds__(256) void k_axpy_dot(T* __restrict__ w, double alpha, const double* __restrict__ alpha_dev,
+                                                  const T* __restrict__ x, const T* __restrict__ y,
+                                                  const uint8_t* __restrict__ owned_lat, int64_t N, int nr, int ncomp,
+                                                  double* __restrict__ part)
 {
     double s = 0.0;
-    const T a = (T)alpha;
+    const T a = alpha_dev ? (T)(alpha * alpha_dev[0]) : (T)alpha;
     const int lane = threadIdx.x & 63;
     for (ColumnWalk cw(N / nr, ncomp); cw.col < cw.ncol; cw.col += cw.step) {
         const bool own = owned_lat[cw.col % cw.nlat];
@@ -376,6 +379,16 @@ __global__ void k_nonfinite(const T* __restrict__ y, int64_t count, int* flag)
     if (bad) atomicOr(flag, 1);
 }
 
+// y = x where the device flag is set (the decision stays on the device)
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_if_flag(T* __restrict__ y, const T* __restrict__ x, int64_t count,
+                                                      const int* __restrict__ flag)
+{
+    if (flag[0] == 0) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = x[i];
+}
+
 // out = max |y[i]| (non-negative doubles order like their bit patterns)
 template <typename T>
 __global__ __launch_bounds__(256) void k_max_abs(const T* __restrict__ y, int64_t count, unsigned long long* out)
@@ -498,18 +511,25 @@ void launch_cheby_update(T* x, T* d, const T* az, const T* b, const T* dinv, int
 
 template <typename T>
 void launch_axpy_dot(const MeshLevel& m, int ncomp, T* w, double alpha, const T* x, const T* y, double* out_dev,
-                     double* part_dev, hipStream_t s)
+                     double* part_dev, hipStream_t s, const double* alpha_dev)
 {
     const int g = reduce_blocks(m, ncomp);
     const dim3 gr(g), bl(256);
     if (y == w)
-        hipLaunchKernelGGL((k_axpy_dot<T, true>), gr, bl, 0, s, w, alpha, x, y, m.d_owned_lat.p, m.N, m.nr, ncomp,
-                           part_dev);
+        hipLaunchKernelGGL((k_axpy_dot<T, true>), gr, bl, 0, s, w, alpha, alpha_dev, x, y, m.d_owned_lat.p, m.N, m.nr,
+                           ncomp, part_dev);
     else
-        hipLaunchKernelGGL((k_axpy_dot<T, false>), gr, bl, 0, s, w, alpha, x, y, m.d_owned_lat.p, m.N, m.nr, ncomp,
-                           part_dev);
+        hipLaunchKernelGGL((k_axpy_dot<T, false>), gr, bl, 0, s, w, alpha, alpha_dev, x, y, m.d_owned_lat.p, m.N, m.nr,
+                           ncomp, part_dev);
     TN_HIP(hipGetLastError());
     launch_reduce_finalize(part_dev, g, 1, out_dev, s);
+}
+
+template <typename T>
+void launch_copy_if_flag(T* y, const T* x, int64_t count, const int* flag_dev, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_if_flag<T>, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, y, x, count, flag_dev);
+    TN_HIP(hipGetLastError());
 }
 
 template <typename T>
@@ -622,7 +642,8 @@ template void launch_cast<float, double>(float*, const double*, int64_t, hipStre
     template void launch_sub_scaled<T>(T*, int64_t, const double*, double, hipStream_t);                         \
     template void launch_sum_owned<T>(const MeshLevel&, const T*, double*, double*, hipStream_t);                 \
     template void launch_axpy_dot<T>(const MeshLevel&, int, T*, double, const T*, const T*, double*, double*,    \
-                                     hipStream_t);                                                               \
+                                     hipStream_t, const double*);                                                \
+    template void launch_copy_if_flag<T>(T*, const T*, int64_t, const int*, hipStream_t);                        \
     template void launch_cheby_update<T>(T*, T*, const T*, const T*, const T*, int64_t, double, double, int,    \
                                          hipStream_t);                                                           \
     template void launch_interface_sum<T>(const MeshLevel&, int, T*, hipStream_t);                               \

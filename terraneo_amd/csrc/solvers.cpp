@@ -47,6 +47,16 @@ Ctx::Ctx()
     red.alloc(32);
     part.alloc(REDUCE_PART_DOUBLES);
     flag.alloc(1);
+    col.alloc(COL_SLOTS);
+}
+
+void Ctx::fetch_col(int n, double* out, int* flag_out)
+{
+    TN_REQUIRE(n >= 1 && n <= COL_SLOTS, "fetch_col: slot count");
+    TN_HIP(hipMemcpyAsync(out, col.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (flag_out) TN_HIP(hipMemcpyAsync(flag_out, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TN_HIP(hipStreamSynchronize(s));
+    check_march_status();
 }
 
 double Ctx::fetch(int slots, double* out)
@@ -125,6 +135,31 @@ double v_axpy_dot(Ctx& c, const BVec& w, double alpha, const BVec& x, const BVec
     return c.fetch(1);
 }
 
+void v_dot_to(Ctx& c, const BVec& a, const BVec& b, double* out_dev)
+{
+    for (int i = 0; i < a.nb; ++i) {
+        const Field& f = a.b[i];
+        if (f.dtype == TN_F64)
+            launch_dot_owned<double>(*f.m, f.ncomp, (const double*)f.p, (const double*)b.b[i].p, out_dev, c.part.p,
+                                     c.s);
+        else
+            launch_dot_owned<float>(*f.m, f.ncomp, (const float*)f.p, (const float*)b.b[i].p, out_dev, c.part.p, c.s);
+    }
+}
+
+void v_sub_axpy_dot_to(Ctx& c, const BVec& w, const double* coef_dev, const BVec& x, const BVec& y, double* out_dev)
+{
+    for (int i = 0; i < w.nb; ++i) {
+        const Field& f = w.b[i];
+        if (f.dtype == TN_F64)
+            launch_axpy_dot<double>(*f.m, f.ncomp, (double*)f.p, -1.0, (const double*)x.b[i].p,
+                                    (const double*)y.b[i].p, out_dev, c.part.p, c.s, coef_dev);
+        else
+            launch_axpy_dot<float>(*f.m, f.ncomp, (float*)f.p, -1.0, (const float*)x.b[i].p, (const float*)y.b[i].p,
+                                   out_dev, c.part.p, c.s, coef_dev);
+    }
+}
+
 void v_multidot(Ctx& c, const BVec& a, const std::vector<const BVec*>& bs, double* out)
 {
     const int n = (int)bs.size();
@@ -180,7 +215,7 @@ void v_randomize(Ctx& c, const BVec& y, uint64_t seed)
     }
 }
 
-bool v_nonfinite(Ctx& c, const BVec& y)
+void v_replace_nonfinite(Ctx& c, const BVec& y, const BVec& x)
 {
     launch_fill<int>(c.flag.p, 1, 0, c.s);
     for (int i = 0; i < y.nb; ++i) {
@@ -190,11 +225,15 @@ bool v_nonfinite(Ctx& c, const BVec& y)
         else
             launch_nonfinite<float>((const float*)f.p, f.count(), c.flag.p, c.s);
     }
-    int h = 0;
-    TN_HIP(hipMemcpyAsync(&h, c.flag.p, sizeof(int), hipMemcpyDeviceToHost, c.s));
-    TN_HIP(hipStreamSynchronize(c.s));
-    check_march_status();
-    return h != 0;
+    for (int i = 0; i < y.nb; ++i) {
+        const Field& f = y.b[i];
+        const Field& g = x.b[i];
+        TN_REQUIRE(f.dtype == g.dtype && f.count() == g.count(), "replace_nonfinite: block mismatch");
+        if (f.dtype == TN_F64)
+            launch_copy_if_flag<double>((double*)f.p, (const double*)g.p, f.count(), c.flag.p, c.s);
+        else
+            launch_copy_if_flag<float>((float*)f.p, (const float*)g.p, f.count(), c.flag.p, c.s);
+    }
 }
 
 void v_convert(Ctx& c, const BVec& dst, const BVec& src)
@@ -489,6 +528,7 @@ FGMRES::FGMRES(KrylovParams p, std::vector<BVec> tmps, Solver* prec, bool skip)
     : p_(p), t_(std::move(tmps)), prec_(prec), skip_nonfinite_(skip)
 {
     TN_REQUIRE((int)t_.size() >= 2 * p_.restart + 4, "FGMRES needs 2*restart+4 temporaries");
+    TN_REQUIRE(p_.restart + 1 <= Ctx::COL_SLOTS, "FGMRES: restart length at most 63");
 }
 
 void FGMRES::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
@@ -522,23 +562,25 @@ void FGMRES::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
         int inner = 0;
         for (int j = 0; j < m && total < p_.max_iterations; ++j, ++total) {
             P.solve_zero(c, A, Z(j), V(j));
-            if (skip_nonfinite_ && v_nonfinite(c, Z(j))) {
-                // skip the preconditioner for this step (fgmres.hpp:180-193)
-                v_assign(c, Z(j), V(j));
-                ++prec_skipped;
-            }
+            // a non-finite preconditioned vector is replaced by V_j, i.e. the
+            // preconditioner is skipped for this step (fgmres.hpp:180-193); the
+            // test and the copy run on the device, the flag is read with the column
+            if (skip_nonfinite_) v_replace_nonfinite(c, Z(j), V(j));
             A.apply(c, Z(j), w);
             // modified Gram-Schmidt: h_ij = w . V_i, w -= h_ij V_i for i = 0..j in
-            // turn; each subtraction is fused with the next dot (last: the norm)
-            double hij = v_dot(c, w, V(0));
-            double hn2 = 0.0;
-            for (int i = 0; i <= j; ++i) {
-                h(i, j) = hij;
-                if (i < j)
-                    hij = v_axpy_dot(c, w, -hij, V(i), V(i + 1));
-                else
-                    hn2 = v_axpy_dot(c, w, -hij, V(i), w);
-            }
+            // turn; each subtraction is fused with the next dot (last: the norm).
+            // The coefficients stay on the device (col[i] = h_ij, col[j+1] = |w|^2),
+            // so the column costs one host synchronisation, not j + 2.
+            double* hc = c.col.p;
+            launch_fill<double>(hc, j + 2, 0.0, c.s);
+            v_dot_to(c, w, V(0), hc);
+            for (int i = 0; i <= j; ++i) v_sub_axpy_dot_to(c, w, hc + i, V(i), i < j ? V(i + 1) : w, hc + i + 1);
+            double hv[Ctx::COL_SLOTS];
+            int skipped = 0;
+            c.fetch_col(j + 2, hv, skip_nonfinite_ ? &skipped : nullptr);
+            if (skipped) ++prec_skipped;
+            for (int i = 0; i <= j; ++i) h(i, j) = hv[i];
+            const double hn2 = hv[j + 1];
             const double hn = std::sqrt(std::max(hn2, 0.0));
             if (hn < eps * r0) {
                 h(j + 1, j) = 0;

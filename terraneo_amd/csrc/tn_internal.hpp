@@ -146,10 +146,14 @@ void launch_dot_owned(const MeshLevel& m, int ncomp, const T* a, const T* b, dou
 template <typename T>
 void launch_freeslip_project(const MeshLevel& m, T* v, int which, hipStream_t s);
 
-// w += alpha x; *out_dev += sum_owned w . y (y == w: squared norm of the new w)
+// w += alpha x; *out_dev += sum_owned w . y (y == w: squared norm of the new w).
+// alpha_dev (optional): the coefficient is alpha * alpha_dev[0], read on the device.
 template <typename T>
 void launch_axpy_dot(const MeshLevel& m, int ncomp, T* w, double alpha, const T* x, const T* y, double* out_dev,
-                     double* part_dev, hipStream_t s);
+                     double* part_dev, hipStream_t s, const double* alpha_dev = nullptr);
+// y = x where flag_dev[0] != 0 (count entries)
+template <typename T>
+void launch_copy_if_flag(T* y, const T* x, int64_t count, const int* flag_dev, hipStream_t s);
 // Chebyshev step: z = dinv (b - az); d = alpha z + beta d; x += d.
 // mode 0: general; 1: first step (d = alpha z); 2: first step from x = 0
 // (az and x not read, x = d written; az may be null)

@@ -51,8 +51,12 @@ struct Ctx {
     DevBuf<double> red;   // reduction slots
     DevBuf<double> part;  // per-block partial sums of the reductions
     DevBuf<int> flag;
+    DevBuf<double> col;   // reduction results kept on the device (a Hessenberg column)
+    static constexpr int COL_SLOTS = 64;
     Ctx();
     double fetch(int slots, double* out = nullptr);
+    // col[0..n) to out, and the flag to *flag_out when given: one host synchronisation
+    void fetch_col(int n, double* out, int* flag_out = nullptr);
 };
 
 // level-1 (all blocks)
@@ -63,10 +67,16 @@ void v_assign(Ctx& c, const BVec& y, const BVec& x);
 double v_dot(Ctx& c, const BVec& a, const BVec& b);
 // w += alpha x, then return w . y (owned nodes; y may be w itself)
 double v_axpy_dot(Ctx& c, const BVec& w, double alpha, const BVec& x, const BVec& y);
+// Device-resident forms (no host synchronisation): *out_dev += a . b, and
+// w -= coef_dev[0] x followed by *out_dev += w . y
+void v_dot_to(Ctx& c, const BVec& a, const BVec& b, double* out_dev);
+void v_sub_axpy_dot_to(Ctx& c, const BVec& w, const double* coef_dev, const BVec& x, const BVec& y, double* out_dev);
+// y = x if y holds a non-finite entry; the test and the copy stay on the device
+// (c.flag holds the outcome until the next fetch_col)
+void v_replace_nonfinite(Ctx& c, const BVec& y, const BVec& x);
 void v_mul(Ctx& c, const BVec& y, const BVec& d);
 void v_invert(Ctx& c, const BVec& y);
 void v_randomize(Ctx& c, const BVec& y, uint64_t seed);
-bool v_nonfinite(Ctx& c, const BVec& y);
 void v_convert(Ctx& c, const BVec& dst, const BVec& src);
 // out[k] = a . bs[k] (k < n <= 16), one pass over a
 void v_multidot(Ctx& c, const BVec& a, const std::vector<const BVec*>& bs, double* out);
