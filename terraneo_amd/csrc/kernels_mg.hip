@@ -48,6 +48,11 @@ __device__ __forceinline__ float vec_get(const float4& v, int i)
     return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
 }
 __device__ __forceinline__ double vec_get(const double2& v, int i) { return i == 0 ? v.x : v.y; }
+// 16-byte matrix vectors in flight per wave in the stored-matrix apply, and
+// the occupancy its register budget is sized for (3 waves per SIMD leave room
+// for the vectors in flight; at 4 the compiler waits for every load)
+constexpr int STORED_PREFETCH = 6;
+constexpr int STORED_WAVES_PER_EU = 3;
 // The matrix stream is read once per apply and is larger than the L2 and the
 // last-level cache: non-temporal loads keep it from evicting the vectors.
 __device__ __forceinline__ float4 ld_stream(const float4* p)
@@ -613,18 +618,24 @@ __device__ __forceinline__ void stored_cell_sums(const StoredArgs<T, S>& a, int6
 #pragma unroll
             for (int i = 0; i < 18; ++i) Y(i) += (T)a.stored[stored_at<S>(cell, w, pk(i, i))] * xs[i];
         } else {
-            // packed entries (row-major upper triangle), V per 16-byte vector load
-            constexpr int V = stored_vec<S>();
+            // packed entries (row-major upper triangle), V per 16-byte vector load.
+            // Vector q is loaded when the arithmetic reaches vector q - PD, so
+            // PD loads per wave are in flight (latency, not bandwidth, bounds a
+            // stream with one load in flight per wave).
+            constexpr int V = stored_vec<S>(), NV = WEDGE_SLOTS / V;
+            constexpr int PD = STORED_PREFETCH;
             using Vec = VecOf<S>;
             const Vec* vp = reinterpret_cast<const Vec*>(a.stored + stored_at<S>(cell, w, 0));
+            Vec buf[NV];
+#pragma unroll
+            for (int q = 0; q < PD; ++q) buf[q] = ld_stream(vp + q * 64);
             int k = 0;
-            Vec cur;
 #pragma unroll
             for (int i = 0; i < 18; ++i) {
 #pragma unroll
                 for (int j = i; j < 18; ++j, ++k) {
-                    if (k % V == 0) cur = ld_stream(vp + (k / V) * 64);
-                    const T aij = (T)vec_get(cur, k % V);
+                    if (k % V == 0 && k / V + PD < NV) buf[k / V + PD] = ld_stream(vp + (k / V + PD) * 64);
+                    const T aij = (T)vec_get(buf[k / V], k % V);
                     Y(i) += aij * xs[j];
                     if (j != i) Y(j) += aij * xs[i];
                 }
@@ -634,7 +645,7 @@ __device__ __forceinline__ void stored_cell_sums(const StoredArgs<T, S>& a, int6
 }
 
 template <typename T, typename S, bool DIAG>
-__global__ __launch_bounds__(256) void k_stored_cells(StoredArgs<T, S> a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STORED_WAVES_PER_EU, STORED_WAVES_PER_EU))) void k_stored_cells(StoredArgs<T, S> a)
 {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
