@@ -184,14 +184,14 @@ def bilinear_derivative(pr, m, u, lam, d, h=1e-4):
     return (vals[0] - vals[1]) / (2 * h)
 
 
-def run(lat=8, rad=7, levels=6, cheby_iterations=2, max_iterations=6000, chunk=240, rtol=1e-8):
+def run(lat=8, rad=7, levels=6, cheby_iterations=2, max_iterations=6000, chunk=240, rtol=1e-8, prec="f32"):
     import numpy as np
     import torch
 
     from . import workloads as W
 
     t0 = time.perf_counter()
-    pr, mt, bu, bp, coords, radii = build_problem(lat, rad, levels, cheby_iterations, rtol)
+    pr, mt, bu, bp, coords, radii = build_problem(lat, rad, levels, cheby_iterations, rtol, prec_dtype=prec)
     vel = pr.vel
     setup = time.perf_counter() - t0
     free, total = torch.cuda.mem_get_info()
@@ -235,20 +235,23 @@ def run(lat=8, rad=7, levels=6, cheby_iterations=2, max_iterations=6000, chunk=2
     n_nodes = vel.N
     y = torch.empty_like(u)
     ms64 = time_apply(pr.A, u, y)
-    u32 = u.float()
-    y32 = torch.empty_like(u32)
-    ms32 = time_apply(pr.ops[-1], u32, y32)
-    del u32, y32
+    ms32 = None
+    if prec == "f32":
+        u32 = u.float()
+        y32 = torch.empty_like(u32)
+        ms32 = time_apply(pr.ops[-1], u32, y32)
+        del u32, y32
     _log(f"apply FP64 {ms64:.3f} ms ({3 * n_nodes / ms64 / 1e6:.1f} GDoF/s, "
-         f"{56 * n_nodes / ms64 / 1e9:.3f} TB/s algorithmic), "
-         f"FP32 {ms32:.3f} ms ({3 * n_nodes / ms32 / 1e6:.1f} GDoF/s)")
+         f"{56 * n_nodes / ms64 / 1e9:.3f} TB/s algorithmic)" +
+         (f", FP32 {ms32:.3f} ms ({3 * n_nodes / ms32 / 1e6:.1f} GDoF/s)" if ms32 else ""))
 
     n = 2 ** lat + 1
     nr = 2 ** rad + 1
     return {
         "workload": f"free-slip Stokes forward + adjoint, velocity lateral level {lat} ({n}x{n} per diamond) x "
-                    f"{nr} radial, {levels}-level FP32 GMG (Chebyshev degree 3 x{cheby_iterations}, FP32 Galerkin "
-                    f"matrices, FP64 dense coarsest) in FP64 FGMRES(30), rel. res {rtol:g}, per-cell viscosity "
+                    f"{nr} radial, {levels}-level {prec.upper()} GMG (Chebyshev degree 3 x{cheby_iterations}, "
+                    f"{prec.upper()} Galerkin matrices, FP64 dense coarsest) in FP64 FGMRES(30), rel. res {rtol:g}, "
+                    "per-cell viscosity "
                     "contrast 1e6 (seed 6) x100 for r > 0.95, penalty once per node",
         "velocity_dof": 3 * n_nodes,
         "setup_seconds": round(setup, 2),
@@ -258,8 +261,9 @@ def run(lat=8, rad=7, levels=6, cheby_iterations=2, max_iterations=6000, chunk=2
         "adjoint_rhs_rotation_content": rot,
         "gradient_kernel_ms": round(grad_ms, 4),
         "gradient_vs_central_difference_of_bilinear_form": float(f"{grad_fd_rel:.3e}"),
-        "apply_ms": {"f64": round(ms64, 4), "f32": round(ms32, 4)},
-        "apply_gdof_per_s": {"f64": round(3 * n_nodes / ms64 / 1e6, 2), "f32": round(3 * n_nodes / ms32 / 1e6, 2)},
+        "apply_ms": {"f64": round(ms64, 4), "f32": round(ms32, 4) if ms32 else None},
+        "apply_gdof_per_s": {"f64": round(3 * n_nodes / ms64 / 1e6, 2),
+                             "f32": round(3 * n_nodes / ms32 / 1e6, 2) if ms32 else None},
         "apply_hbm_fraction_f64": round(56 * n_nodes / ms64 / 1e9 / 8.0, 4),
         "device_memory_after_setup_gib": round(used_setup, 1),
     }
@@ -273,11 +277,12 @@ def main(argv=None):
     ap.add_argument("--cheby-iterations", type=int, default=2)
     ap.add_argument("--max-iterations", type=int, default=6000)
     ap.add_argument("--chunk", type=int, default=240)
+    ap.add_argument("--prec", choices=("f32", "f64"), default="f32", help="V-cycle precision")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
     if a.chunk % 30:
         sys.exit("--chunk must be a multiple of the restart length 30")
-    out = run(a.lat, a.rad, a.levels, a.cheby_iterations, a.max_iterations, a.chunk)
+    out = run(a.lat, a.rad, a.levels, a.cheby_iterations, a.max_iterations, a.chunk, prec=a.prec)
     line = json.dumps(out)
     print(line, flush=True)
     if a.json:
