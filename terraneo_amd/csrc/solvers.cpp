@@ -47,12 +47,12 @@ Ctx::Ctx()
     red.alloc(32);
     part.alloc(REDUCE_PART_DOUBLES);
     flag.alloc(1);
-    col.alloc(COL_SLOTS);
+    col.alloc(64);
 }
 
 void Ctx::fetch_col(int n, double* out, int* flag_out)
 {
-    TN_REQUIRE(n >= 1 && n <= COL_SLOTS, "fetch_col: slot count");
+    TN_REQUIRE(n >= 1 && (size_t)n <= col.n, "fetch_col: slot count");
     TN_HIP(hipMemcpyAsync(out, col.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     if (flag_out) TN_HIP(hipMemcpyAsync(flag_out, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
     TN_HIP(hipStreamSynchronize(s));
@@ -528,7 +528,6 @@ FGMRES::FGMRES(KrylovParams p, std::vector<BVec> tmps, Solver* prec, bool skip)
     : p_(p), t_(std::move(tmps)), prec_(prec), skip_nonfinite_(skip)
 {
     TN_REQUIRE((int)t_.size() >= 2 * p_.restart + 4, "FGMRES needs 2*restart+4 temporaries");
-    TN_REQUIRE(p_.restart + 1 <= Ctx::COL_SLOTS, "FGMRES: restart length at most 63");
 }
 
 void FGMRES::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
@@ -548,7 +547,8 @@ void FGMRES::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
     info.initial_residual = info.final_residual = r0;
     info.history.push_back(r0);
     if (beta0 <= p_.atol) return;
-    std::vector<double> H((m + 1) * m), g(m + 1), cs(m + 1), sn(m + 1), y(m);
+    std::vector<double> H((m + 1) * m), g(m + 1), cs(m + 1), sn(m + 1), y(m), hv(m + 2);
+    if (c.col.n < (size_t)m + 2) c.col.alloc(m + 2);
     auto h = [&](int i, int j) -> double& { return H[i * m + j]; };
     const double eps = std::numeric_limits<double>::epsilon();
     int total = 0;
@@ -575,9 +575,8 @@ void FGMRES::solve(Ctx& c, LinOp& A, const BVec& x, const BVec& b)
             launch_fill<double>(hc, j + 2, 0.0, c.s);
             v_dot_to(c, w, V(0), hc);
             for (int i = 0; i <= j; ++i) v_sub_axpy_dot_to(c, w, hc + i, V(i), i < j ? V(i + 1) : w, hc + i + 1);
-            double hv[Ctx::COL_SLOTS];
             int skipped = 0;
-            c.fetch_col(j + 2, hv, skip_nonfinite_ ? &skipped : nullptr);
+            c.fetch_col(j + 2, hv.data(), skip_nonfinite_ ? &skipped : nullptr);
             if (skipped) ++prec_skipped;
             for (int i = 0; i <= j; ++i) h(i, j) = hv[i];
             const double hn2 = hv[j + 1];

@@ -51,8 +51,7 @@ struct Ctx {
     DevBuf<double> red;   // reduction slots
     DevBuf<double> part;  // per-block partial sums of the reductions
     DevBuf<int> flag;
-    DevBuf<double> col;   // reduction results kept on the device (a Hessenberg column)
-    static constexpr int COL_SLOTS = 64;
+    DevBuf<double> col;   // reduction results kept on the device (a Hessenberg column; grown on demand)
     Ctx();
     double fetch(int slots, double* out = nullptr);
     // col[0..n) to out, and the flag to *flag_out when given: one host synchronisation

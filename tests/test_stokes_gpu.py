@@ -94,7 +94,7 @@ def test_mass_operators_match_oracle(cuda):
     assert np.abs(d.cpu().numpy() - refd).max() <= 1e-13 * np.abs(refd).max()
 
 
-def assess_solve_gpu(cuda, level, min_level=2, kind="zeroslip"):
+def assess_solve_gpu(cuda, level, min_level=2, kind="zeroslip", restart=30):
     """Reference assess tests on the GPU: EpsDivDivStokes, FGMRES(30) rtol 1e-14,
     block-triangular preconditioner, V-cycle x2 with Chebyshev(2) x3, constant
     transfers, re-discretised coarse operators, PCG coarsest (1000, 1e-6),
@@ -152,7 +152,7 @@ def assess_solve_gpu(cuda, level, min_level=2, kind="zeroslip"):
     tn.api.kmass_lumped(mp, torch.ones(mp.N, **f64), mass)
     u, p = torch.zeros_like(f), torch.zeros_like(zp)
     info = tn.stokes_solve(K, tn.TN_FGMRES_BLOCKTRI, mg, mass, u, p, bu, bp, max_iterations=200, rtol=1e-14,
-                           atol=1e-16, restart=30)
+                           atol=1e-16, restart=restart)
     own_u = np.tile(mv.owned_mask().reshape(-1).astype(bool), 3)
     own_p = mp.owned_mask().reshape(-1).astype(bool)
     uu, pp, ue, pe = u.cpu().numpy(), p.cpu().numpy(), uex.cpu().numpy(), pex.cpu().numpy()
@@ -183,6 +183,17 @@ def test_assess_freeslip_solve_matches_reference_errors(cuda, kind, level):
     rv, rp = ref[level]
     print(f"{kind} level {level}: vel {ev:.6e} (ref {rv}) pre {ep:.6e} (ref {rp}) its {info['iterations']} "
           f"res {info['final_residual'] / info['initial_residual']:.2e} t {info['seconds']:.2f}s")
+    assert info["final_residual"] / info["initial_residual"] < 1e-12
+    assert abs(ev - rv) / rv < 1e-4, (ev, rv)
+    assert abs(ep - rp) / rp < 1e-4, (ep, rp)
+
+
+@pytest.mark.gpu
+def test_assess_solve_long_restart(cuda):
+    """FGMRES(100): the Hessenberg column kept on the device grows past its initial 64 slots"""
+    ev, ep, info = assess_solve_gpu(cuda, 3, restart=100)
+    rv, rp = AS.REFERENCE_ERRORS_DIRICHLET[3]
+    print(f"restart 100: vel {ev:.6e} (ref {rv}) pre {ep:.6e} (ref {rp}) its {info['iterations']}")
     assert info["final_residual"] / info["initial_residual"] < 1e-12
     assert abs(ev - rv) / rv < 1e-4, (ev, rv)
     assert abs(ep - rp) / rp < 1e-4, (ep, rp)
