@@ -507,56 +507,74 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3))) voi
     march::run<T, W>(op, a.nf, a.nrf, a.seg, a.guard);
 }
 
+// One row sum of a wedge's k-weighted mass matrix (kmass.hpp): vertex j of
+// wedge w of hex cell (d, x, y, rc), the reference's 6-point quadrature.
+template <typename T>
+__device__ double kmass_wedge_row(const MassArgs<T>& a, int d, int x, int y, int rc, int w, int jrow)
+{
+    WedgeGeo G;
+    load_wedge(a.coords, a.radii, a.n, d, x, y, rc, w, G);
+    int64_t idx[6];
+    for (int j = 0; j < 6; ++j)
+        idx[j] = (((int64_t)d * a.n + x + WOFF[w][j][0]) * a.n + y + WOFF[w][j][1]) * a.nr + rc + WOFF[w][j][2];
+    const double gr = 0.5 * (G.r2 - G.r1);
+    double M[6][6] = {{0}};
+    for (int q = 0; q < 6; ++q) {
+        const double* qp = Q6P[q];
+        double Jl[3][3];  // lateral Jacobian (kernel_helpers.hpp:153-176)
+        for (int c = 0; c < 3; ++c) {
+            Jl[c][0] = G.P[1][c] - G.P[0][c];
+            Jl[c][1] = G.P[2][c] - G.P[0][c];
+            Jl[c][2] = (1 - qp[0] - qp[1]) * G.P[0][c] + qp[0] * G.P[1][c] + qp[1] * G.P[2][c];
+        }
+        const double det = fabs(Jl[0][0] * (Jl[1][1] * Jl[2][2] - Jl[1][2] * Jl[2][1]) -
+                                Jl[0][1] * (Jl[1][0] * Jl[2][2] - Jl[1][2] * Jl[2][0]) +
+                                Jl[0][2] * (Jl[1][0] * Jl[2][1] - Jl[1][1] * Jl[2][0]));
+        const double rr = G.r1 + 0.5 * (G.r2 - G.r1) * (1 + qp[2]);
+        double N6[6];
+        for (int j = 0; j < 6; ++j) N6[j] = shape_lat(j, qp[0], qp[1]) * shape_rad(j, qp[2]);
+        double kq = 0.0;
+        for (int j = 0; j < 6; ++j) kq += N6[j] * (double)a.k[idx[j]];
+        const double f = Q6W * kq * rr * rr * gr * det;
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) M[i][j] += f * N6[i] * N6[j];
+    }
+    double s = 0.0;
+    for (int j = 0; j < 6; ++j) s += M[jrow][j];
+    return s;
+}
+
 // lumped k-weighted mass (kmass.hpp): row sums of the k-weighted wedge mass
-// matrices, accumulated per wedge (setup only)
+// matrices, as a gather (setup only): one thread per node copy adds the row
+// sums of the wedges of its diamond that contain it, in a fixed order, so the
+// result repeats bit for bit (the Schur-complement diagonal of the solves)
 template <typename T>
 __global__ __launch_bounds__(128) void k_kmass_lumped(MassArgs<T> a)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ncr = a.nr - 1, ncl = a.n - 1;
-    const int64_t ncell = 10LL * ncl * ncl * ncr;
-    if (tid >= ncell) return;
-    const int rc = (int)(tid % ncr);
-    int64_t lat = tid / ncr;
-    const int y = (int)(lat % ncl);
-    lat /= ncl;
-    const int x = (int)(lat % ncl);
-    const int d = (int)(lat / ncl);
-    const int64_t N = 10LL * a.n * a.n * a.nr;
-    for (int w = 0; w < 2; ++w) {
-        WedgeGeo G;
-        load_wedge(a.coords, a.radii, a.n, d, x, y, rc, w, G);
-        int64_t idx[6];
-        for (int j = 0; j < 6; ++j)
-            idx[j] = (((int64_t)d * a.n + x + WOFF[w][j][0]) * a.n + y + WOFF[w][j][1]) * a.nr + rc + WOFF[w][j][2];
-        const double gr = 0.5 * (G.r2 - G.r1);
-        double M[6][6] = {{0}};
-        for (int q = 0; q < 6; ++q) {
-            const double* qp = Q6P[q];
-            double Jl[3][3];  // lateral Jacobian (kernel_helpers.hpp:153-176)
-            for (int c = 0; c < 3; ++c) {
-                Jl[c][0] = G.P[1][c] - G.P[0][c];
-                Jl[c][1] = G.P[2][c] - G.P[0][c];
-                Jl[c][2] = (1 - qp[0] - qp[1]) * G.P[0][c] + qp[0] * G.P[1][c] + qp[1] * G.P[2][c];
+    const int n = a.n, nr = a.nr, ncr = nr - 1, ncl = n - 1;
+    if (tid >= 10LL * n * n * nr) return;
+    const int lev = (int)(tid % nr);
+    int64_t lat = tid / nr;
+    const int Y = (int)(lat % n);
+    lat /= n;
+    const int X = (int)(lat % n);
+    const int d = (int)(lat / n);
+    double acc = 0.0;
+    for (int rc = lev - 1; rc <= lev; ++rc) {
+        if (rc < 0 || rc >= ncr) continue;
+        for (int qx = X - 1; qx <= X; ++qx) {
+            if (qx < 0 || qx >= ncl) continue;
+            for (int qy = Y - 1; qy <= Y; ++qy) {
+                if (qy < 0 || qy >= ncl) continue;
+                for (int w = 0; w < 2; ++w)
+                    for (int j = 0; j < 6; ++j)
+                        if (qx + WOFF[w][j][0] == X && qy + WOFF[w][j][1] == Y && rc + WOFF[w][j][2] == lev)
+                            acc += kmass_wedge_row(a, d, qx, qy, rc, w, j);
             }
-            const double det = fabs(Jl[0][0] * (Jl[1][1] * Jl[2][2] - Jl[1][2] * Jl[2][1]) -
-                                    Jl[0][1] * (Jl[1][0] * Jl[2][2] - Jl[1][2] * Jl[2][0]) +
-                                    Jl[0][2] * (Jl[1][0] * Jl[2][1] - Jl[1][1] * Jl[2][0]));
-            const double rr = G.r1 + 0.5 * (G.r2 - G.r1) * (1 + qp[2]);
-            double N6[6];
-            for (int j = 0; j < 6; ++j) N6[j] = shape_lat(j, qp[0], qp[1]) * shape_rad(j, qp[2]);
-            double kq = 0.0;
-            for (int j = 0; j < 6; ++j) kq += N6[j] * (double)a.k[idx[j]];
-            const double f = Q6W * kq * rr * rr * gr * det;
-            for (int i = 0; i < 6; ++i)
-                for (int j = 0; j < 6; ++j) M[i][j] += f * N6[i] * N6[j];
-        }
-        for (int i = 0; i < 6; ++i) {
-            double s = 0.0;
-            for (int j = 0; j < 6; ++j) s += M[i][j];
-            unsafeAtomicAdd(a.y + idx[i], (T)s);
         }
     }
+    a.y[tid] = (T)acc;
 }
 
 // VectorMass as a gather: one thread per node copy writes y_i = sum over the
@@ -657,8 +675,8 @@ template <typename T>
 void launch_mass(const MassArgs<T>& a, bool lumped_k, hipStream_t s)
 {
     if (lumped_k) {
-        const int64_t ncell = 10LL * (a.n - 1) * (a.n - 1) * (a.nr - 1);
-        hipLaunchKernelGGL((k_kmass_lumped<T>), dim3((unsigned)((ncell + 127) / 128)), dim3(128), 0, s, a);
+        const int64_t N = 10LL * a.n * a.n * a.nr;
+        hipLaunchKernelGGL((k_kmass_lumped<T>), dim3((unsigned)((N + 127) / 128)), dim3(128), 0, s, a);
     } else {
         TN_REQUIRE(a.ncomp >= 1 && a.ncomp <= 3, "mass: ncomp must be 1..3");
         const int64_t N = 10LL * a.n * a.n * a.nr;

@@ -128,7 +128,7 @@ def surface_objective(pr, u):
     return J, g
 
 
-def _rotation_projection(pr, v):
+def rotation_projection(pr, v):
     """P v, the owned-inner-product projection of v onto the rigid rotations
     e_k x x (the fs/fs near-null space the rotation penalty controls)"""
     import numpy as np
@@ -148,10 +148,10 @@ def _rotation_projection(pr, v):
 
 
 def rotation_content(pr, v):
-    """|P v| / |v| (see _rotation_projection)"""
+    """|P v| / |v| (see rotation_projection)"""
     import numpy as np
 
-    pv = _rotation_projection(pr, v)
+    pv = rotation_projection(pr, v)
     return float(np.sqrt(pr.vel.dot(pv, pv, 3) / pr.vel.dot(v, v, 3)))
 
 
@@ -163,7 +163,7 @@ def remove_rotations(pr, g):
     the viscosity gradient, is zero.  Removing P g leaves the gradient unchanged
     and takes the slowly converging, penalty-scaled component out of the
     adjoint solve."""
-    return (g - _rotation_projection(pr, g)).contiguous()
+    return (g - rotation_projection(pr, g)).contiguous()
 
 
 def bilinear_derivative(pr, m, u, lam, d, h=1e-4):
