@@ -39,7 +39,7 @@ def parse():
     p.add_argument("--solve-lat", type=int, default=7)
     p.add_argument("--solve-rad", type=int, default=7)
     p.add_argument("--solve-levels", type=int, default=5)
-    p.add_argument("--solve-max-iterations", type=int, default=2500)
+    p.add_argument("--solve-max-iterations", type=int, default=4000)
     p.add_argument("--solve-cheby-iterations", type=int, default=2, help="Chebyshev (degree 3) sweeps per smoothing")
     p.add_argument("--no-tight-reference", dest="tight_reference", action="store_false",
                    help="skip the FP64 solve continued to 1e-12 that the velocity errors are measured against")
