@@ -89,9 +89,26 @@ def solve_bench(args):
         torch.cuda.synchronize()
         setup = time.perf_counter() - t0
         log(f"solve {prec}: setup {setup:.2f} s")
-        u, p, info = pr.solve(bu, bp, rtol=1e-8, max_iterations=args.solve_max_iterations, restart=30,
-                              pressure_mean_free=True)
-        log(f"solve {prec}: {info['iterations']} iterations, {info['seconds']:.3f} s")
+        # whole FGMRES(30) cycles in calls of 600 iterations, warm-started, each
+        # printing a progress line: a call boundary is a restart, which recomputes
+        # the true residual exactly as a restart inside one call does, so the
+        # iterates are those of a single call; the stopping test keeps 1e-8 of
+        # the first call's initial residual
+        u = torch.zeros(3 * pr.vel.N, dtype=torch.float64, device=dev)
+        p = torch.zeros(pr.pre.N, dtype=torch.float64, device=dev)
+        total, secs, r0 = 0, 0.0, None
+        while total < args.solve_max_iterations:
+            n_it = min(600, args.solve_max_iterations - total)
+            kw = dict(rtol=1e-8, atol=1e-300) if r0 is None else dict(rtol=1e-300, atol=1e-8 * r0)
+            u, p, step = pr.solve(bu, bp, max_iterations=n_it, restart=30, u=u, p=p, pressure_mean_free=True, **kw)
+            if r0 is None:
+                r0 = step["initial_residual"]
+            total += step["iterations"]
+            secs += step["seconds"]
+            log(f"solve {prec}: {total} iterations, rel. res {step['final_residual'] / r0:.3e}, {secs:.1f} s")
+            if step["final_residual"] <= 1e-8 * r0 or step["iterations"] < n_it:
+                break
+        info = {"iterations": total, "seconds": secs, "initial_residual": r0}
         res = pr.residual(u, p, bu, bp, pressure_mean_free=True)
         out[prec] = {"seconds": round(info["seconds"], 4), "iterations": info["iterations"],
                      "ms_per_iteration": round(info["seconds"] / max(info["iterations"], 1) * 1e3, 3),
