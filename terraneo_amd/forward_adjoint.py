@@ -37,9 +37,9 @@ def _log(msg):
     print(msg, flush=True)
 
 
-def chunked_solve(pr, bu, bp, rtol, max_iterations, chunk, label):
-    """FGMRES(30) solve from zero in calls of `chunk` iterations (a multiple of
-    the restart length), stopping at rtol times the first residual."""
+def chunked_solve(pr, bu, bp, rtol, max_iterations, chunk, label, restart=30):
+    """FGMRES(restart) solve from zero in calls of `chunk` iterations (a multiple
+    of the restart length), stopping at rtol times the first residual."""
     import torch
 
     u = torch.zeros(3 * pr.vel.N, **pr.f64)
@@ -48,7 +48,7 @@ def chunked_solve(pr, bu, bp, rtol, max_iterations, chunk, label):
     while total < max_iterations:
         n = min(chunk, max_iterations - total)
         kw = dict(rtol=rtol, atol=1e-300) if r0 is None else dict(rtol=1e-300, atol=rtol * r0)
-        u, p, info = pr.solve(bu, bp, max_iterations=n, restart=30, u=u, p=p, pressure_mean_free=True, **kw)
+        u, p, info = pr.solve(bu, bp, max_iterations=n, restart=restart, u=u, p=p, pressure_mean_free=True, **kw)
         calls += 1
         if r0 is None:
             r0 = info["initial_residual"]
@@ -166,6 +166,58 @@ def remove_rotations(pr, g):
     return (g - rotation_projection(pr, g)).contiguous()
 
 
+def residual_report(pr, u, p, bu, bp, label):
+    """Where the residual of a solve sits: velocity and pressure parts, the
+    velocity part on the two shells (and its normal component there) and inside,
+    its rigid-rotation content, |r_u|^2 and |r_p|^2 by radial eighth, and the
+    cell viscosity at the largest velocity residual entries.  Printed lines."""
+    import numpy as np
+    import torch
+
+    vel, pre = pr.vel, pr.pre
+    own_p = torch.from_numpy(pre.owned_mask().reshape(-1).astype(np.float64)).to(u.device)
+    p_mf = p - (own_p * p).sum() / own_p.sum()
+    ku, kp = torch.empty_like(u), torch.empty_like(p)
+    pr.K.apply(u, p_mf, ku, kp)
+    ru = (bu - ku).contiguous()
+    rp = bp - kp
+    rp = (rp - (own_p * rp).sum() / own_p.sum()).contiguous()
+
+    def nrm(v, nc=3, m=vel):
+        v = v.contiguous()
+        return float(np.sqrt(m.dot(v, v, nc)))
+
+    nb = nrm(bu)
+    _log(f"[{label} residual] |b_u| {nb:.3e}, |r_u| {nrm(ru):.3e}, |r_p| {nrm(rp, 1, pre):.3e}")
+    R = ru.view(3, 10, vel.n, vel.n, vel.nr)
+    c = torch.from_numpy(np.asarray(vel.lateral_coords())).to(u.device).permute(3, 0, 1, 2)
+    for name, lev in (("CMB", 0), ("surface", vel.nr - 1)):
+        z = torch.zeros_like(R)
+        z[..., lev] = R[..., lev]
+        zn = torch.zeros_like(R)
+        zn[..., lev] = (R[..., lev] * c).sum(0)[None] * c
+        _log(f"[{label} residual] r_u on the {name} level {nrm(z.view(-1)):.3e} (normal part {nrm(zn.view(-1)):.3e})")
+    zi = R.clone()
+    zi[..., 0] = 0
+    zi[..., -1] = 0
+    _log(f"[{label} residual] r_u inside {nrm(zi.view(-1)):.3e}, rigid-rotation content {rotation_content(pr, ru):.3e}")
+    prof = np.array_split((R ** 2).sum(dim=(0, 1, 2, 3)).cpu().numpy(), 8)
+    _log(f"[{label} residual] |r_u|^2 by radial eighth (CMB to surface): " + " ".join(f"{b.sum():.2e}" for b in prof))
+    P = rp.view(10, pre.n, pre.n, pre.nr)
+    pp = np.array_split((P ** 2).sum(dim=(0, 1, 2)).cpu().numpy(), 8)
+    _log(f"[{label} residual] |r_p|^2 by radial eighth: " + " ".join(f"{b.sum():.2e}" for b in pp))
+    if pr.eta_cell is not None:
+        e = pr.eta_cell.view(10, vel.n - 1, vel.n - 1, vel.nr - 1)
+        mag = (R ** 2).sum(0).view(-1)
+        for i in torch.topk(mag, 3).indices.tolist():
+            d, rem = divmod(i, vel.n * vel.n * vel.nr)
+            x, rem = divmod(rem, vel.n * vel.nr)
+            y, r = divmod(rem, vel.nr)
+            ec = e[d, min(x, vel.n - 2), min(y, vel.n - 2), min(r, vel.nr - 2)].item()
+            _log(f"[{label} residual] large r_u {mag[i].sqrt().item():.3e} at diamond {d} ({x}, {y}) level {r}, "
+                 f"cell viscosity {ec:.2e}")
+
+
 def bilinear_derivative(pr, m, u, lam, d, h=1e-4):
     """central difference of l^T A(m) u along d (two applies at m +- h d)"""
     import torch
@@ -184,7 +236,8 @@ def bilinear_derivative(pr, m, u, lam, d, h=1e-4):
     return (vals[0] - vals[1]) / (2 * h)
 
 
-def run(lat=8, rad=7, levels=6, cheby_iterations=2, max_iterations=6000, chunk=240, rtol=1e-8, prec="f32"):
+def run(lat=8, rad=7, levels=6, cheby_iterations=2, max_iterations=6000, chunk=240, rtol=1e-8, prec="f32",
+        report=False, restart=30):
     import numpy as np
     import torch
 
@@ -199,18 +252,22 @@ def run(lat=8, rad=7, levels=6, cheby_iterations=2, max_iterations=6000, chunk=2
     _log(f"setup {setup:.1f} s: velocity {3 * vel.N} DoF, pressure {pr.pre.N} DoF, "
          f"device memory in use {used_setup:.1f} GiB")
 
-    u, p, fwd = chunked_solve(pr, bu, bp, rtol, max_iterations, chunk, "forward")
+    u, p, fwd = chunked_solve(pr, bu, bp, rtol, max_iterations, chunk, "forward", restart)
     fwd["true_rel_residual"] = pr.residual(u, p, bu, bp, pressure_mean_free=True)
     _log(f"forward: true rel. res {fwd['true_rel_residual']:.3e}")
+    if report:
+        residual_report(pr, u, p, bu, bp, "forward")
 
     J, g = surface_objective(pr, u)
     rot = rotation_content(pr, g)
     g = remove_rotations(pr, g)
     _log(f"adjoint right-hand side: rigid-rotation content {rot:.3e} (removed; it cannot change the gradient)")
     gp = torch.zeros(pr.pre.N, **pr.f64)
-    lam, q, adj = chunked_solve(pr, g, gp, rtol, max_iterations, chunk, "adjoint")
+    lam, q, adj = chunked_solve(pr, g, gp, rtol, max_iterations, chunk, "adjoint", restart)
     adj["true_rel_residual"] = pr.residual(lam, q, g, gp, pressure_mean_free=True)
     _log(f"adjoint: true rel. res {adj['true_rel_residual']:.3e}")
+    if report:
+        residual_report(pr, lam, q, g, gp, "adjoint")
     del q, gp
 
     grad = torch.empty_like(mt)
@@ -250,7 +307,7 @@ def run(lat=8, rad=7, levels=6, cheby_iterations=2, max_iterations=6000, chunk=2
     return {
         "workload": f"free-slip Stokes forward + adjoint, velocity lateral level {lat} ({n}x{n} per diamond) x "
                     f"{nr} radial, {levels}-level {prec.upper()} GMG (Chebyshev degree 3 x{cheby_iterations}, "
-                    f"{prec.upper()} Galerkin matrices, FP64 dense coarsest) in FP64 FGMRES(30), rel. res {rtol:g}, "
+                    f"{prec.upper()} Galerkin matrices, FP64 dense coarsest) in FP64 FGMRES({restart}), rel. res {rtol:g}, "
                     "per-cell viscosity "
                     "contrast 1e6 (seed 6) x100 for r > 0.95, penalty once per node",
         "velocity_dof": 3 * n_nodes,
@@ -278,11 +335,14 @@ def main(argv=None):
     ap.add_argument("--max-iterations", type=int, default=6000)
     ap.add_argument("--chunk", type=int, default=240)
     ap.add_argument("--prec", choices=("f32", "f64"), default="f32", help="V-cycle precision")
+    ap.add_argument("--residual-report", action="store_true", help="where each solve's final residual sits")
+    ap.add_argument("--restart", type=int, default=30, help="FGMRES restart length (the workload's: 30)")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
-    if a.chunk % 30:
-        sys.exit("--chunk must be a multiple of the restart length 30")
-    out = run(a.lat, a.rad, a.levels, a.cheby_iterations, a.max_iterations, a.chunk, prec=a.prec)
+    if a.chunk % a.restart:
+        sys.exit("--chunk must be a multiple of the restart length")
+    out = run(a.lat, a.rad, a.levels, a.cheby_iterations, a.max_iterations, a.chunk, prec=a.prec,
+              report=a.residual_report, restart=a.restart)
     line = json.dumps(out)
     print(line, flush=True)
     if a.json:
