@@ -556,9 +556,6 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
     const __amdgpu_buffer_rsrc_t rs_eta = buf_rsrc(nodal ? (const void*)a.eta : (const void*)a.src, nodal ? Nb : 0u);
     const int bmode = a.bc_cmb, tmode = a.bc_surf;
     const int nchunks = (ncr + 63) >> 6;
-    // fs/fs rotation penalty y += gamma x p, gamma = S beta (FP64, formed by the
-    // moment pass's finalize, which runs before this kernel on the same stream)
-    const double pg0 = PEN ? pen[13] : 0.0, pg1 = PEN ? pen[14] : 0.0, pg2 = PEN ? pen[15] : 0.0;
     // byte offsets of radial columns: node (xc, yc) of this diamond at ((d n + xc) n + yc) nr
     const uint32_t colstep = (uint32_t)nr * SZ;             // next y
     const uint32_t rowstep = (uint32_t)n * nr * SZ;         // next x
@@ -643,16 +640,6 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
             const int step = step0 + (qy - qs);
             T(*E)[65] = ebuf[step % RING][w];
             T yb[4][3], yt[4][3];
-            // fs/fs penalty: unit coordinates of the node (x, qy) stored by this
-            // step, loaded (wave-uniform address) before the cell arithmetic hides
-            // their latency
-            double pc0 = 0.0, pc1 = 0.0, pc2 = 0.0;
-            if (PEN && row_own && qy >= y0) {
-                const int o = __builtin_amdgcn_readfirstlane(3 * ((d * n + x) * n + qy));
-                pc0 = coords_d[o];
-                pc1 = coords_d[o + 1];
-                pc2 = coords_d[o + 2];
-            }
             if (quad) {
                 const uint32_t cb = colx + (uint32_t)(qy + 1) * colstep;
                 if (!PF) {
@@ -673,6 +660,23 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
                 for (int k = 0; k < 4; ++k)
 #pragma unroll
                     for (int j = 0; j < 3; ++j) yb[k][j] = yt[k][j] = T(0);
+            }
+            // fs/fs penalty y += gamma x p = r (gamma x c): the wave-uniform vector
+            // w = gamma x c for the node (x, qy) stored by this step (c its unit
+            // position), formed after the cell arithmetic, whose geometry holds the
+            // scalar registers; the merge, the LDS routing and the hand-off waits
+            // hide the loads' latency.  gamma = S beta is re-read from the moment
+            // pass's finalize output (a scalar-cache hit) rather than kept for the
+            // whole kernel.
+            double pw0 = 0.0, pw1 = 0.0, pw2 = 0.0;
+            if (PEN && row_own && qy >= y0) {
+                asm volatile("" ::: "memory");
+                const int o = __builtin_amdgcn_readfirstlane(3 * ((d * n + x) * n + qy));
+                const double c0 = coords_d[o], c1 = coords_d[o + 1], c2 = coords_d[o + 2];
+                const double g0 = pen[13], g1 = pen[14], g2 = pen[15];
+                pw0 = g1 * c2 - g2 * c1;
+                pw1 = g2 * c0 - g0 * c2;
+                pw2 = g0 * c1 - g1 * c0;
             }
             if (w + 1 < W) march::wait_counter(&used[w + 1], step - RING + 1, a.guard);
             T part[3], tp[3] = {0, 0, 0};
@@ -721,10 +725,9 @@ __global__ __launch_bounds__(64 * W) void k_eps_flow(MarchArgs<T> a, const T* __
                         }
                         if (PEN) {
                             const double r = level == c ? rad_lane : rad_top;
-                            const double px = pc0 * r, py = pc1 * r, pz = pc2 * r;
-                            v.x += (T)(pg1 * pz - pg2 * py);
-                            v.y += (T)(pg2 * px - pg0 * pz);
-                            v.z += (T)(pg0 * py - pg1 * px);
+                            v.x += (T)(r * pw0);
+                            v.y += (T)(r * pw1);
+                            v.z += (T)(r * pw2);
                         }
                     }
                     const uint32_t lo = (uint32_t)level * SZ;
