@@ -45,7 +45,7 @@ struct MarchArgs {
     const double* coords_d;   // FP64 lateral positions where the rotation penalty is added (0: not added)
     const double* radii_d;
     const T* blk;             // [2][10 n n][3] boundary quadratic forms, or null (both shells Neumann)
-    const double* pen;        // fs/fs penalty: [0..3) beta, [4..13) S, or null
+    const double* pen;        // fs/fs penalty: [0..3) beta, [4..13) S, [13..16) gamma = S beta, or null
     int n, nr;
     int seg;                  // node columns per block in the march direction
     int bc_cmb, bc_surf;

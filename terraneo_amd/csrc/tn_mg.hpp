@@ -48,7 +48,7 @@ struct StoredArgs {
     T* csum;     // [12][cells] merged corner sums (pass 1 -> pass 2)
     T* ctop;     // [12][10 (n-1)^2 chunks] chunk-top level sums
     const T* blk;        // [2][10 n n][3] boundary-node quadratic forms (launch_stored_blk), or null
-    const double* pen;   // fs/fs penalty: [0..3) beta, [4..13) S, or null
+    const double* pen;   // fs/fs penalty: [0..3) beta, [4..13) S, [13..16) gamma = S beta, or null
     const double* radii; // [nr] (penalty positions)
     const double* pen_coords;  // [10][n][n][3] lateral positions where the penalty is added (0: not added)
 };
