@@ -30,7 +30,7 @@ using tn::MeshLevel;
 tn::FvArgs tn_fv::args() const
 {
     const MeshLevel& L = mesh->lvl;
-    return tn::FvArgs{L.d_coords.p, L.d_radii.p, centers.p, n, nr};
+    return tn::FvArgs{L.d_coords.p, L.d_radii.p, centers.p, columns.p, n, nr};
 }
 
 namespace {
@@ -115,6 +115,7 @@ int tn_fv_create(tn_mesh* m, tn_fv** out)
         fv->nr = L.nr;
         fv->size = tn::fv_size(L.n, L.nr);
         fv->centers.alloc(3 * fv->size);
+        fv->columns.alloc((size_t)tn::FV_COLUMN * 10 * (L.n - 1) * (L.n - 1));
         fv->TL.alloc(fv->size);
         fv->Rp.alloc(fv->size);
         fv->Rm.alloc(fv->size);
@@ -131,6 +132,7 @@ int tn_fv_create(tn_mesh* m, tn_fv** out)
         TN_HIP(hipMemset(fv->Rp.p, 0, sizeof(double) * fv->size));
         TN_HIP(hipMemset(fv->Rm.p, 0, sizeof(double) * fv->size));
         TN_HIP(hipMemset(fv->ad.p, 0, sizeof(double) * fv->ad.n));
+        tn::launch_fv_columns(fv->args(), fv->columns.p, nullptr);
         tn::launch_fv_centers(fv->args(), fv->centers.p, nullptr);
         ghost_update(fv, fv->centers.p, 3, nullptr);
         TN_HIP(hipStreamSynchronize(nullptr));

@@ -118,19 +118,69 @@ __device__ __forceinline__ void cross3(const double a[3], const double b[3], dou
     o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+// Lateral part of the geometry of hex column (d, x, y), the same at every
+// radius: g[0..12) the lateral face normals Vb x Va of faces 0..3, g[12..15)
+// and g[15..18) the two triangles' (V1 - V0) x (V2 - V0), g[18] the sum of the
+// wedges' |det(V0, V1, V2)|.  Corners c00 = (x, y), c10 = (x+1, y),
+// c01 = (x, y+1), c11 = (x+1, y+1); wedge 0: V0 = c00, V1 = c10, V2 = c01;
+// wedge 1: W0 = c11, W1 = c01, W2 = c10.  Lateral faces (face, Va, Vb):
+// 0 (c00, c01) xi = 0 of wedge 0: n ~ V2 x V0 = c01 x c00; 2 (c00, c10)
+// eta = 0 of wedge 0: V1 x V0 = c10 x c00; 1 (c11, c10): W2 x W0 = c10 x c11;
+// 3 (c11, c01): W1 x W0 = c01 x c11.
+__device__ __forceinline__ void column_geometry(const double* __restrict__ coords, int n, int d, int x, int y,
+                                                double g[FV_COLUMN])
+{
+    double c[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double* p = coords + 3 * (((int64_t)d * n + x + (k & 1)) * n + y + (k >> 1));
+        c[k][0] = p[0];
+        c[k][1] = p[1];
+        c[k][2] = p[2];
+    }
+    const int FA[4] = {0, 3, 0, 3}, FB[4] = {2, 1, 1, 2};  // corner indices of faces 0, 1, 2, 3
+#pragma unroll
+    for (int f = 0; f < 4; ++f) cross3(c[FB[f]], c[FA[f]], g + 3 * f);
+    double e1[3], e2[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        e1[j] = c[1][j] - c[0][j];
+        e2[j] = c[2][j] - c[0][j];
+    }
+    cross3(e1, e2, g + 12);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        e1[j] = c[2][j] - c[3][j];
+        e2[j] = c[1][j] - c[3][j];
+    }
+    cross3(e1, e2, g + 15);
+    // |det(V0, V1, V2)| per wedge (= V0 . (V1 x V2))
+    double v12[3], w12[3];
+    cross3(c[1], c[2], v12);
+    cross3(c[2], c[1], w12);
+    const double det0 = fabs(c[0][0] * v12[0] + c[0][1] * v12[1] + c[0][2] * v12[2]);
+    const double det1 = fabs(c[3][0] * w12[0] + c[3][1] * w12[1] + c[3][2] * w12[2]);
+    g[18] = det0 + det1;
+    g[19] = 0.0;
+}
+
+// Geometry of real FV cell (d, xc, yc, rc) from its column's lateral part
+// (a.columns, see column_geometry) and the radii.
 __device__ void cell_geometry(const FvArgs& a, const double* __restrict__ vel, int64_t N, int d, int xc, int yc,
                               int rc, double beta[6], double& M, double S[6][3])
 {
     const int n = a.n, nr = a.nr;
     const int x = xc - 1, y = yc - 1, r = rc - 1;
-    // lateral corners: c00 = (x, y), c10 = (x+1, y), c01 = (x, y+1), c11 = (x+1, y+1)
-    double c[4][3];
+    double g[FV_COLUMN];
+    {
+        const double2* gp =
+            reinterpret_cast<const double2*>(a.columns + FV_COLUMN * (((int64_t)d * (n - 1) + x) * (n - 1) + y));
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const double* p = a.coords + 3 * (((int64_t)d * n + x + (k & 1)) * n + y + (k >> 1));
-        c[k][0] = p[0];
-        c[k][1] = p[1];
-        c[k][2] = p[2];
+        for (int k = 0; k < FV_COLUMN / 2; ++k) {
+            const double2 v = gp[k];
+            g[2 * k] = v.x;
+            g[2 * k + 1] = v.y;
+        }
     }
     // nodal velocity at the 4 corners x {bottom, top}
     double ub[4][3], ut[4][3];
@@ -146,15 +196,10 @@ __device__ void cell_geometry(const FvArgs& a, const double* __restrict__ vel, i
     const double r1 = a.radii[r], r2 = a.radii[r + 1];
     const double h = 0.5 * (r2 - r1);
     const double ab = (2.0 * r1 + r2) / 3.0, at = (r1 + 2.0 * r2) / 3.0;
-    // wedge 0: V0 = c00, V1 = c10, V2 = c01; wedge 1: W0 = c11, W1 = c01, W2 = c10
-    // lateral faces (face, Va, Vb): 0 (c00, c01) xi = 0 of wedge 0: n ~ V2 x V0 = c01 x c00;
-    // 2 (c00, c10) eta = 0 of wedge 0: V1 x V0 = c10 x c00; 1 (c11, c10): W2 x W0 = c10 x c11;
-    // 3 (c11, c01): W1 x W0 = c01 x c11
     const int FA[4] = {0, 3, 0, 3}, FB[4] = {2, 1, 1, 2};  // corner indices of faces 0, 1, 2, 3
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
-        double nv[3];
-        cross3(c[FB[f]], c[FA[f]], nv);
+        const double* nv = g + 3 * f;
         const double sS = h * (r1 + r2);
         double w = 0;
 #pragma unroll
@@ -166,19 +211,8 @@ __device__ void cell_geometry(const FvArgs& a, const double* __restrict__ vel, i
         S[f][2] = sS * nv[2];
     }
     // radial faces: both wedges' (V1 - V0) x (V2 - V0)
-    double e1[3], e2[3], n0[3], n1[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        e1[j] = c[1][j] - c[0][j];
-        e2[j] = c[2][j] - c[0][j];
-    }
-    cross3(e1, e2, n0);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        e1[j] = c[2][j] - c[3][j];
-        e2[j] = c[1][j] - c[3][j];
-    }
-    cross3(e1, e2, n1);
+    const double* n0 = g + 12;
+    const double* n1 = g + 15;
     double bb = 0, bt = 0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -207,13 +241,7 @@ __device__ void cell_geometry(const FvArgs& a, const double* __restrict__ vel, i
             beta[f] = -beta[f];
         }
     }
-    // volume: |det(V0, V1, V2)| per wedge (= V0 . (V1 x V2))
-    double v12[3], w12[3];
-    cross3(c[1], c[2], v12);
-    cross3(c[2], c[1], w12);
-    const double det0 = fabs(c[0][0] * v12[0] + c[0][1] * v12[1] + c[0][2] * v12[2]);
-    const double det1 = fabs(c[3][0] * w12[0] + c[3][1] * w12[1] + c[3][2] * w12[2]);
-    M = (det0 + det1) * (r2 - r1) * (r1 * r1 + r1 * r2 + r2 * r2) / 6.0;  // (r2^3 - r1^3) / 6 without cancellation
+    M = g[18] * (r2 - r1) * (r1 * r1 + r1 * r2 + r2 * r2) / 6.0;  // (r2^3 - r1^3) / 6 without cancellation
 }
 
 __device__ __forceinline__ int64_t neighbour(int n, int nr, int d, int xc, int yc, int rc, int f)
@@ -224,6 +252,20 @@ __device__ __forceinline__ int64_t neighbour(int n, int nr, int d, int xc, int y
 }
 
 // ---------------------------------------------------------------- kernels
+
+// lateral geometry of every hex column (column_geometry), one thread per column
+__global__ __launch_bounds__(FV_BLOCK) void k_fv_columns(FvArgs a, double* __restrict__ columns)
+{
+    const int n = a.n;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 10LL * (n - 1) * (n - 1)) return;
+    const int y = (int)(t % (n - 1)), x = (int)((t / (n - 1)) % (n - 1)), d = (int)(t / ((int64_t)(n - 1) * (n - 1)));
+    double g[FV_COLUMN];
+    column_geometry(a.coords, n, d, x, y, g);
+    double2* o = reinterpret_cast<double2*>(columns + FV_COLUMN * t);
+#pragma unroll
+    for (int k = 0; k < FV_COLUMN / 2; ++k) o[k] = make_double2(g[2 * k], g[2 * k + 1]);
+}
 
 // cell centre = (int_K x) / |K| with the Felippa rule on both wedges (helpers.hpp:123-175)
 __global__ __launch_bounds__(FV_BLOCK) void k_fv_centers(FvArgs a, double* __restrict__ c)
@@ -497,6 +539,12 @@ int fv_dt_blocks(int n, int nr) { return (int)std::min<int64_t>(blocks_for(fv_re
 void launch_fv_centers(const FvArgs& a, double* c, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fv_centers, dim3(blocks_for(fv_real_cells(a.n, a.nr))), dim3(FV_BLOCK), 0, s, a, c);
+}
+
+void launch_fv_columns(const FvArgs& a, double* columns, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fv_columns, dim3(blocks_for(10LL * (a.n - 1) * (a.n - 1))), dim3(FV_BLOCK), 0, s, a,
+                       columns);
 }
 
 void launch_fv_ghost(const int32_t* pairs, int cnt, int nr, int ncomp, double* f, hipStream_t s)
