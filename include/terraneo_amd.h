@@ -356,8 +356,9 @@ int tn_stokes_solve(tn_stokes* K, int method, tn_gmg* prec_u, const void* schur_
  * field of that mesh, [3][10][n][n][nr].  Results repeat bit for bit (no
  * atomics). */
 /* FV state on the velocity mesh: cell centres with their ghost layers
- * (fv/hex/helpers.hpp:177-190 initialize_cell_centers) and the FCT work
- * arrays (FVFCTBuffers, fct_advection_diffusion.hpp:27-76) */
+ * (fv/hex/helpers.hpp:177-190 initialize_cell_centers), the FCT work
+ * arrays (FVFCTBuffers, fct_advection_diffusion.hpp:27-76) and a table of the
+ * lateral cell geometry per hex column (160 bytes per column) */
 int tn_fv_create(tn_mesh* m, tn_fv** out);
 int tn_fv_destroy(tn_fv* fv);
 /* FV scalar elements including ghosts: 10 (n+1)^2 (nr+1) */
