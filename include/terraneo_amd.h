@@ -399,6 +399,15 @@ int tn_fct_upwind_step(tn_fv* fv, double* T, const double* vel, double dt, doubl
 /* right-hand side of the FV -> FE L2 projection, b_i = sum_K T_K int_K phi_i,
  * interface summed (nodal scalar of the velocity mesh) */
 int tn_fv_to_fe_rhs(tn_fv* fv, const double* Tfv, double* b, tn_stream_t s);
+/* bound-preserving FV -> FE transfer through the lumped mass
+ * (conversion.hpp:242-341, l2_project_fv_to_fe_lumped): Tfe_i = b_i / D_i with
+ * b_i = sum_K T_K int_K phi_i and D_i = int phi_i; tmp: one nodal FP64 scratch
+ * vector (receives D), distinct from Tfe */
+int tn_fv_to_fe_lumped(tn_fv* fv, const double* Tfv, double* Tfe, double* tmp, tn_stream_t s);
+/* FE -> FV cell averages T_K = int_K u_h / |K| with the Felippa rule, lateral
+ * ghost layers updated (conversion.hpp:348-416, l2_project_fe_to_fv); the
+ * radial ghost cells at the shells are left as they are */
+int tn_fe_to_fv(tn_fv* fv, const double* Tfe, double* Tfv, tn_stream_t s);
 /* L2 projection FV -> FE (l2_project_fv_to_fe, fv/hex/conversion.hpp:131-211):
  * CG on the consistent mass matrix, no preconditioner, from Tfe's content;
  * the reference uses 1000 iterations and rtol = atol = 1e-12 */

@@ -127,6 +127,8 @@ SIGNATURES = {
     "tn_fct_explicit_step": (i32, [vp, vp, vp, f64, f64, vp, i32, i32, f64, i32, f64, vp]),
     "tn_fct_upwind_step": (i32, [vp, vp, vp, f64, f64, vp, i32, vp]),
     "tn_fv_to_fe_rhs": (i32, [vp, vp, vp, vp]),
+    "tn_fv_to_fe_lumped": (i32, [vp, vp, vp, vp, vp]),
+    "tn_fe_to_fv": (i32, [vp, vp, vp, vp]),
     "tn_fv_to_fe": (i32, [vp, vp, vp, vp, vp, vp, i32, vp]),
     "tn_xdmf_create": (i32, [vp, C.c_char_p, i32, i32, pvp]),
     "tn_xdmf_destroy": (i32, [vp]),

@@ -454,6 +454,16 @@ class FV:
         check(lib().tn_fv_to_fe_rhs(self.h, _ptr(Tfv), _ptr(b), _stream_ptr(stream)))
         return b
 
+    def to_fe_lumped(self, Tfv, Tfe, tmp, stream=None):
+        """bound-preserving FV -> nodal transfer b / D through the lumped mass (tmp receives D)"""
+        check(lib().tn_fv_to_fe_lumped(self.h, _ptr(Tfv), _ptr(Tfe), _ptr(tmp), _stream_ptr(stream)))
+        return Tfe
+
+    def from_fe(self, Tfe, Tfv, stream=None):
+        """FE -> FV cell averages, lateral ghost layers updated"""
+        check(lib().tn_fe_to_fv(self.h, _ptr(Tfe), _ptr(Tfv), _stream_ptr(stream)))
+        return Tfv
+
     def to_fe(self, Tfv, Tfe, max_iterations=1000, rtol=1e-12, atol=1e-12, stream=None):
         """L2 projection FV -> nodal (CG on the mass matrix from Tfe's content)"""
         o = KrylovOptions(max_iterations, rtol, atol, 30, 0)

@@ -268,6 +268,29 @@ int tn_fct_upwind_step(tn_fv* fv, double* T, const double* vel, double dt, doubl
     TN_API_END
 }
 
+int tn_fv_to_fe_lumped(tn_fv* fv, const double* Tfv, double* Tfe, double* tmp, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(fv && Tfv && Tfe && tmp, "null pointer");
+    TN_REQUIRE(Tfe != tmp, "tn_fv_to_fe_lumped: Tfe and tmp must be different arrays");
+    const hipStream_t hs = (hipStream_t)s;
+    tn::launch_fv_to_fe_rhs(fv->args(), Tfv, Tfe, hs);
+    tn::launch_interface_sum<double>(fv->mesh->lvl, 1, Tfe, hs);
+    tn::launch_fv_to_fe_rhs(fv->args(), nullptr, tmp, hs);
+    tn::launch_interface_sum<double>(fv->mesh->lvl, 1, tmp, hs);
+    tn::launch_fv_lumped_scale(tmp, Tfe, 10LL * fv->n * fv->n * fv->nr, hs);
+    TN_API_END
+}
+
+int tn_fe_to_fv(tn_fv* fv, const double* Tfe, double* Tfv, tn_stream_t s)
+{
+    TN_API_BEGIN
+    TN_REQUIRE(fv && Tfe && Tfv, "null pointer");
+    tn::launch_fe_to_fv(fv->args(), Tfe, Tfv, (hipStream_t)s);
+    ghost_update(fv, Tfv, 1, (hipStream_t)s);
+    TN_API_END
+}
+
 int tn_fv_to_fe_rhs(tn_fv* fv, const double* Tfv, double* b, tn_stream_t s)
 {
     TN_API_BEGIN

@@ -486,6 +486,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_fct_correction(int n, int nr, cons
 // int_K phi_i |det J| with the Felippa 3x2 rule, which is exact: per wedge
 //   |det(V0, V1, V2)| / 6 x (I_bot or I_top),  I = int rho^2 x radial hat drho,
 //   I_bot = dr (r1^2/2 + r1 dr/3 + dr^2/12),  I_top = dr (r1^2/2 + 2 r1 dr/3 + dr^2/4)
+// Tfv null: every T_K = 1, which gives the lumped diagonal D_i = int phi_i.
 __global__ __launch_bounds__(FV_BLOCK) void k_fv_to_fe_rhs(FvArgs a, const double* __restrict__ Tfv,
                                                            double* __restrict__ b)
 {
@@ -519,11 +520,59 @@ __global__ __launch_bounds__(FV_BLOCK) void k_fv_to_fe_rhs(FvArgs a, const doubl
                 const double r1 = a.radii[cr], dr = a.radii[cr + 1] - r1;
                 const double irad = (r == cr + 1) ? dr * (0.5 * r1 * r1 + r1 * dr * (2.0 / 3.0) + 0.25 * dr * dr)
                                                   : dr * (0.5 * r1 * r1 + r1 * dr / 3.0 + dr * dr / 12.0);
-                acc += wlat * irad * Tfv[fvi(n, nr, d, cx + 1, cy + 1, cr + 1)];
+                acc += wlat * irad * (Tfv ? Tfv[fvi(n, nr, d, cx + 1, cy + 1, cr + 1)] : 1.0);
             }
         }
     }
     b[t] = acc;
+}
+
+// u_i = b_i (1 / D_i): the lumped FV -> FE transfer's last step
+// (conversion.hpp:337-340, invert_entries then scale_in_place)
+__global__ __launch_bounds__(FV_BLOCK) void k_fv_lumped_scale(const double* __restrict__ D, double* __restrict__ u,
+                                                             int64_t N)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) u[t] = u[t] * (1.0 / D[t]);
+}
+
+// FE -> FV cell averages (conversion.hpp:348-416): T_K = int_K u_h / |K| with
+// the Felippa 3x2 rule on both wedges; wedge shape i = lateral hat (1 - xi -
+// eta, xi, eta)[i % 3] x radial hat ((1 - zeta) / 2 for i < 3, else (1 + zeta) / 2)
+__global__ __launch_bounds__(FV_BLOCK) void k_fe_to_fv(FvArgs a, const double* __restrict__ u,
+                                                       double* __restrict__ Tfv)
+{
+    const int n = a.n, nr = a.nr;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 10LL * (n - 1) * (n - 1) * (nr - 1)) return;
+    int d, xc, yc, rc;
+    cell_real(t, n, nr, d, xc, yc, rc);
+    const int x = xc - 1, y = yc - 1, r = rc - 1;
+    double P[2][3][3];
+    wedge_corners(a.coords, n, d, x, y, P);
+    const double r1 = a.radii[r], r2 = a.radii[r + 1];
+    double vol = 0, integral = 0;
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        double lu[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+            lu[i] = u[(((int64_t)d * n + x + FV_WOFF[w][i][0]) * n + y + FV_WOFF[w][i][1]) * nr + r + FV_WOFF[w][i][2]];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            double xi, eta, zeta;
+            qpoint(q, xi, eta, zeta);
+            const double Jw = abs_det(P[w], r1, r2, xi, eta, zeta) * QW;
+            vol += Jw;
+            const double lat[3] = {1 - xi - eta, xi, eta};
+            const double bot = 0.5 * (1 - zeta), top = 0.5 * (1 + zeta);
+            double uq = 0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) uq += lu[i] * lat[i % 3] * (i < 3 ? bot : top);
+            integral += uq * Jw;
+        }
+    }
+    Tfv[fvi(n, nr, d, xc, yc, rc)] = integral / vol;
 }
 
 int blocks_for(int64_t work) { return (int)((work + FV_BLOCK - 1) / FV_BLOCK); }
@@ -586,6 +635,16 @@ void launch_fct_correction(int n, int nr, const double* TL, const double* ad, co
 {
     hipLaunchKernelGGL(k_fct_correction, dim3(blocks_for(fv_real_cells(n, nr))), dim3(FV_BLOCK), 0, s, n, nr, TL,
                        ad, fv_size(n, nr), Rp, Rm, T);
+}
+
+void launch_fv_lumped_scale(const double* D, double* u, int64_t N, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fv_lumped_scale, dim3(blocks_for(N)), dim3(FV_BLOCK), 0, s, D, u, N);
+}
+
+void launch_fe_to_fv(const FvArgs& a, const double* u, double* Tfv, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fe_to_fv, dim3(blocks_for(fv_real_cells(a.n, a.nr))), dim3(FV_BLOCK), 0, s, a, u, Tfv);
 }
 
 void launch_fv_to_fe_rhs(const FvArgs& a, const double* Tfv, double* b, hipStream_t s)

@@ -35,7 +35,9 @@ void launch_fv_dirichlet(int n, int nr, double* f, int apply_cmb, double t_cmb, 
 void launch_fct_limiter(int n, int nr, const double* TL, const double* ad, double* Rp, double* Rm, hipStream_t s);
 void launch_fct_correction(int n, int nr, const double* TL, const double* ad, const double* Rp, const double* Rm,
                            double* T, hipStream_t s);
-void launch_fv_to_fe_rhs(const FvArgs& a, const double* Tfv, double* b, hipStream_t s);
+void launch_fv_to_fe_rhs(const FvArgs& a, const double* Tfv, double* b, hipStream_t s);  // Tfv null: all ones
+void launch_fv_lumped_scale(const double* D, double* u, int64_t N, hipStream_t s);      // u *= 1 / D
+void launch_fe_to_fv(const FvArgs& a, const double* u, double* Tfv, hipStream_t s);    // real cells only
 
 }  // namespace tn
 

@@ -170,3 +170,69 @@ def test_fv_to_fe_projection(cuda, lat, radii, nr):
     x1 = torch.zeros(gm.N, dtype=torch.float64, device=cuda)
     fv.to_fe(one, x1)
     assert float((x1 - 1).abs().max()) <= 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lat,radii,nr", SHAPES)
+def test_fv_to_fe_lumped(cuda, lat, radii, nr):
+    """b / D against the oracle's right-hand side (pinned above); D = M 1, the
+    consistent mass's row sums (partition of unity); constants are kept; the
+    result stays inside the FV values' bounds (a convex combination)."""
+    import torch
+
+    from terraneo_amd import api
+
+    print(f"case lumped fv->fe lat={lat} nr={nr}", flush=True)
+    gm, om = _meshes(lat, radii, nr)
+    fv = api.FV(gm)
+    T = np.random.default_rng(11).uniform(0.2, 0.7, F.fv_shape(om))
+    u = torch.empty(gm.N, dtype=torch.float64, device=cuda)
+    D = torch.empty(gm.N, dtype=torch.float64, device=cuda)
+    fv.to_fe_lumped(torch.from_numpy(T).to(cuda), u, D)
+    Do = F.fv_to_fe_rhs(om, np.ones(F.fv_shape(om)))
+    assert _rel(D.cpu().numpy(), Do) <= 1e-13
+    M1 = O.vector_mass(om, np.concatenate([np.ones(gm.N), np.zeros(2 * gm.N)]))[: gm.N]
+    assert _rel(Do, M1) <= 1e-12
+    uo = F.fv_to_fe_rhs(om, T) * (1.0 / Do)
+    un = u.cpu().numpy()
+    assert _rel(un, uo) <= 1e-13
+    Tr = _real(T)
+    assert un.min() >= Tr.min() - 1e-14 and un.max() <= Tr.max() + 1e-14
+    one = torch.ones(F.fv_shape(om), dtype=torch.float64, device=cuda)
+    fv.to_fe_lumped(one, u, D)
+    assert float((u - 1).abs().max()) <= 1e-14
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lat,radii,nr", SHAPES)
+def test_fe_to_fv_cell_averages(cuda, lat, radii, nr):
+    """The nodal coordinate fields average to the cell centres (their Q1
+    interpolant is the coordinate itself, so T_K = int_K x / |K|, the centre the
+    oracle computes); constants are kept; lateral ghosts equal the exchange;
+    linearity in the nodal field."""
+    import torch
+
+    from terraneo_amd import api
+
+    print(f"case fe->fv lat={lat} nr={nr}", flush=True)
+    gm, om = _meshes(lat, radii, nr)
+    fv = api.FV(gm)
+    X = om.node_xyz()
+    co = F.cell_centers(om)
+    for k in range(3):
+        Tg = torch.zeros(F.fv_shape(om), dtype=torch.float64, device=cuda)
+        fv.from_fe(torch.from_numpy(np.ascontiguousarray(X[..., k]).reshape(-1)).to(cuda), Tg)
+        Tn = Tg.cpu().numpy()
+        assert np.abs(_real(Tn) - _real(co[..., k])).max() <= 1e-14
+        assert np.array_equal(Tn, F.ghost_update(om, Tn.copy(), 1))
+    Tg = torch.zeros(F.fv_shape(om), dtype=torch.float64, device=cuda)
+    fv.from_fe(torch.full((gm.N,), 3.0, dtype=torch.float64, device=cuda), Tg)
+    assert np.abs(_real(Tg.cpu().numpy()) - 3.0).max() <= 1e-14
+    rng = np.random.default_rng(12)
+    a, b = rng.uniform(-1, 1, gm.N), rng.uniform(-1, 1, gm.N)
+    outs = []
+    for v in (a, b, 2.0 * a - b):
+        Tg = torch.zeros(F.fv_shape(om), dtype=torch.float64, device=cuda)
+        fv.from_fe(torch.from_numpy(v).to(cuda), Tg)
+        outs.append(_real(Tg.cpu().numpy()))
+    assert np.abs(outs[2] - (2.0 * outs[0] - outs[1])).max() <= 1e-13
